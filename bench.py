@@ -1,0 +1,131 @@
+"""Benchmark: expanded edges/s of a full exact LRX BFS on S_13 on one MI355X.
+
+A step is one complete BFS of the LRX Cayley graph of S_13 from the identity
+(6,227,020,800 states, 79 layers, 3 generators -> 18,681,062,400 expanded
+edges) through libmgbfs_hip.so, with the engine's device buffers allocated
+beforehand (engine reset + layer loop are inside the timed region).  Every
+step's layer table is checked against the reference's golden S13 table.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
+runs one BFS per GPU and has no multi-GPU path.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+REFERENCE_EDGES_PER_S = 18681062400 / 11.716894  # 8x H200 median, docs/validation/lrx13-eight-h200.md:26
+
+
+def cpu_baseline(n: int, seconds_target: float) -> dict:
+    """The oracle's packed Lehmer-bitmap BFS on this host's cores, on a
+    bounded prefix of the same S_n LRX BFS (stops after the layer at which
+    the edge budget is reached)."""
+    from oracle import oracle as O
+    cores = min(os.cpu_count() or 1, 16)
+    budget = int(seconds_target * 40e6)  # ~40 M edges/s on 16 cores
+    t0 = time.perf_counter()
+    r = O.packed_bfs(n, threads=cores, stop_after_edges=budget)
+    dt = time.perf_counter() - t0
+    return {"value": r["edges"] / dt, "unit": "edges/s", "cores": cores, "kind": "port",
+            "sample": f"S{n} LRX BFS prefix: first {len(r['layers'])} layers, {r['edges']} edges expanded "
+                      f"in {dt:.2f} s (oracle/mgbfs_oracle.c orc_packed_bfs, {cores} threads)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=13)
+    ap.add_argument("--backend", default="permutation_bitmap")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "single-GPU build: one exact BFS runs on one GPU; "
+                                   "multi-GPU sharding is not part of this build", "n_gpus": args.gpus}))
+        return 2
+
+    from multigpubfs_amd import NativeEngine, lrx
+    golden = json.loads((ROOT / "tests" / "golden" / "lrx_layers.json").read_text())["layers"].get(str(args.n))
+    n = args.n
+    states = math.factorial(n)
+    edges = 3 * states
+    engine = NativeEngine(lrx(n), backend=args.backend)
+    print(f"[bench] engine {engine.backend_name} S{n} LRX; warmup {args.warmup}", file=sys.stderr, flush=True)
+    exact = True
+    for _ in range(args.warmup):
+        engine.reset()
+        engine.run()
+        sizes = engine.layer_sizes()
+        exact &= (sizes == golden) if golden else sum(sizes) == states
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None
+    sync()
+    kernel_s = 0.0
+    launches = 0
+    algo_bytes = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        engine.reset()
+        status = engine.run()
+        info = engine.info()
+        kernel_s += info["kernel_seconds"]
+        launches += info["kernel_launches"]
+        algo_bytes += info["algorithmic_bytes"]
+        sizes = engine.layer_sizes()
+        exact &= status == 1 and ((sizes == golden) if golden else sum(sizes) == states)
+    sync()
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    value = args.steps * edges / elapsed
+    avg_launch_s = kernel_s / max(1, launches)
+    bytes_per_launch = algo_bytes / max(1, launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    result = {
+        "metric": f"expanded edges/s on S{n} LRX BFS",
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REFERENCE_EDGES_PER_S if n == 13 else None,
+        "dtype": "u64",
+        "data": "synthetic: implicit Cayley graph (LRX generators on S_n from the identity), no dataset",
+        "config": {"workload": f"S{n} LRX full exact BFS, one GPU", "backend": engine.backend_name,
+                   "states": states, "layers": len(sizes), "edges_per_step": edges,
+                   "peak_layer": max(sizes), "layer_table_exact": bool(exact),
+                   "device_bytes": info["device_bytes"],
+                   "baseline": "reference 8x H200, 11.716894 s (docs/validation/lrx13-eight-h200.md:26)"},
+        "roofline": {"bound": "hbm", "kernel": "bitmap_expand_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches_per_step": launches // max(1, args.steps),
+                     "avg_launch_ms": 1e3 * avg_launch_s, "bytes_per_launch": bytes_per_launch,
+                     "bytes_model": "8F frontier read + 8N next-layer write per layer (bitmap probes excluded)"},
+    }
+    engine.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0 if exact else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

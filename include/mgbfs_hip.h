@@ -1,0 +1,145 @@
+/*
+ * mgbfs_hip.h — C ABI of libmgbfs_hip.so, the MI355X (gfx950) exact BFS
+ * engine for implicit Cayley graphs.
+ *
+ * This library is the drop-in replacement for what the reference's thin FFI
+ * crate crates/mgbfs-cuda binds (libmgbfs_cuda.so), plus the host runtime the
+ * reference keeps in crates/mgbfs-runtime.  Conventions follow the
+ * reference's ABI (cuda/mgbfs_cuda.h:10-18, cuda/generic_visited.h:13-21):
+ *
+ *  - Return value 0 = success.  For kernel-level entry points (those taking a
+ *    `void* stream`) 0 means "successfully enqueued", not completion; a
+ *    nonzero value is a hipError_t (hipErrorInvalidValue on argument
+ *    violations).  `stream` is a hipStream_t cast to void* (NULL = default).
+ *  - Kernel-level entry points never allocate and never synchronise; every
+ *    buffer is caller-owned device memory.  `error` is a caller-zeroed sticky
+ *    device word; bits: 1 table full, 2 visited capacity, 4 future capacity,
+ *    8 bad row reference, 32 count above bound, 64 bucket above LDS capacity,
+ *    128 state outside the packed/rank domain.  On any error bit the future
+ *    layer is invalid and the arena must not be resumed.
+ *  - Engine entry points (mgbfs_engine_*) own their device memory, run on
+ *    their own stream, and are not reentrant: one engine per host thread.
+ *
+ * Packed states: a permutation state of degree n <= 16 whose values are all
+ * < 16 is the uint64 key  sum(state[i] << 4*i)  (gpu/cayley_bfs.cu:50-56,
+ * scripts/state_layout.py:9-16).  A generator acts as child[i] = parent[g[i]]
+ * (multigpubfs/graph_definition.py:74).
+ */
+#ifndef MGBFS_HIP_H
+#define MGBFS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGBFS_HIP_ABI_VERSION 1u
+
+/* Engine status values returned by mgbfs_engine_advance(). */
+#define MGBFS_LAYER 0    /* a new non-empty layer was produced          */
+#define MGBFS_COMPLETE 1 /* frontier exhausted: the BFS is complete      */
+#define MGBFS_RESOURCE 2 /* device capacity reached: INCOMPLETE RESOURCE */
+
+/* Backends (mgbfs_engine_options.backend). */
+#define MGBFS_BACKEND_AUTO 0
+#define MGBFS_BACKEND_BITMAP 1   /* exact Lehmer-rank permutation bitmap          */
+#define MGBFS_BACKEND_SORTED 2   /* sort-unique + three rotating sorted banks     */
+#define MGBFS_BACKEND_SEGMENTED 3 /* sort-unique + segmented open-addressing hash */
+#define MGBFS_BACKEND_GENERIC 4  /* SoA int64/u8 states, all-visited exact hash   */
+
+/* Graph description: the schema-2 graph JSON of the reference
+ * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are
+ * HOST pointers, consumed before mgbfs_engine_create returns. */
+typedef struct mgbfs_graph_desc {
+  uint32_t kind;        /* 0 permutation, 1 matrix                          */
+  uint32_t elements;    /* state length (degree, or rows*cols)              */
+  uint32_t rows, cols;  /* matrix shape (kind 1)                            */
+  uint32_t generators;  /* number of generators                             */
+  uint32_t inverse_closed; /* 1 if every generator's inverse is in the set
+                              (graph_definition.py:84-108); selects three-bank
+                              retention, else full visited history          */
+  const uint32_t* permutations; /* kind 0: [generators][elements]           */
+  const int64_t* matrices;      /* kind 1: [generators][rows*rows]          */
+  const uint32_t* moduli;       /* kind 1: [generators], 0 = no reduction   */
+  const int64_t* start;         /* [elements]                               */
+} mgbfs_graph_desc;
+
+typedef struct mgbfs_engine_options {
+  int32_t device;        /* HIP device ordinal                                 */
+  uint32_t backend;      /* MGBFS_BACKEND_*                                    */
+  uint64_t capacity;     /* max states per layer (0 = plan from free VRAM)     */
+  uint64_t vram_budget;  /* bytes the engine may allocate (0 = free - reserve) */
+  uint32_t flags;        /* reserved, 0                                        */
+  uint32_t reserved;
+} mgbfs_engine_options;
+
+typedef struct mgbfs_engine_info {
+  uint32_t backend;        /* backend actually used                          */
+  uint32_t history_layers; /* 3 (inverse-closed) or 0 (full history)         */
+  uint32_t state_bytes;    /* 8 for packed keys                              */
+  uint32_t depth;          /* index of the current layer                     */
+  uint64_t current_count;  /* states in the current layer                    */
+  uint64_t total_states;   /* states discovered so far                       */
+  uint64_t device_bytes;   /* device memory held by the engine               */
+  uint64_t peak_layer;     /* largest layer so far                           */
+  uint64_t candidates;     /* generated children so far (edges expanded)     */
+  uint64_t unique_candidates; /* distinct children after in-layer dedup     */
+  double kernel_seconds;   /* summed device time of the hot kernels          */
+  uint64_t kernel_launches;   /* launches of the hot kernels                 */
+  uint64_t algorithmic_bytes; /* modelled streaming HBM bytes of those kernels */
+} mgbfs_engine_info;
+
+/* ------------------------------------------------------------ engine */
+/* Host runtime of the single-GPU path; replaces GenericNativeBfs
+ * (crates/mgbfs-runtime/src/generic_native.rs:58-145: automatic_plan, new,
+ * advance, sample, previous_small_sample) and SegmentedDistributedBfs::advance
+ * at world = 1 (segmented_distributed_native.rs:1116-1315). */
+int mgbfs_engine_create(const mgbfs_graph_desc* graph, const mgbfs_engine_options* options, void** engine,
+                        char* error_text, size_t error_capacity);
+/* Expands the current layer.  Returns MGBFS_LAYER / MGBFS_COMPLETE /
+ * MGBFS_RESOURCE (fatal bits in *fatal) or a negative value on a host/HIP
+ * failure.  *next_count receives the size of the new current layer. */
+int mgbfs_engine_advance(void* engine, uint64_t* next_count, uint32_t* fatal);
+/* Runs advance() until completion, a resource stop, max_layers layers or
+ * max_seconds (0 = unlimited).  Returns the last advance() status. */
+int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_seconds);
+int mgbfs_engine_query(void* engine, mgbfs_engine_info* info);
+/* Clears the visited state and re-seeds the start state, keeping every
+ * allocation (repeat runs of the same graph, e.g. timing loops). */
+int mgbfs_engine_reset(void* engine);
+/* Layer sizes so far, depth 0 first (the report.json `layer_sizes`). */
+int mgbfs_engine_layer_sizes(void* engine, uint64_t* sizes, uint32_t capacity, uint32_t* count);
+/* Copies up to `limit` states of the current layer (which = 0) or of the
+ * previous layer (which = 1) as int64 rows [limit][elements] in host memory. */
+int mgbfs_engine_sample(void* engine, uint32_t which, uint64_t limit, int64_t* rows, uint64_t* count);
+/* Packed backends: copies the current layer's packed keys (host memory). */
+int mgbfs_engine_layer_keys(void* engine, uint64_t* keys, uint64_t capacity, uint64_t* count);
+const char* mgbfs_engine_backend_name(void* engine);
+void mgbfs_engine_destroy(void* engine);
+
+/* -------------------------------------------------- packed bitmap kernels */
+/* Seeds the start key: sets its rank bit and writes frontier[0].
+ * Replaces mgbfs_bitmap_seed (cuda/permutation_bitmap.cu:87-99). */
+int mgbfs_packed_bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits,
+                             uint64_t* frontier, uint32_t* error, void* stream);
+/* One fused layer: children of frontier[0..count) under `generators`
+ * permutations (host table [generators][degree]), Lehmer rank, bit claim,
+ * append of newly claimed children to next[] (atomic *next_count).
+ * Replaces mgbfs_bitmap_route + mgbfs_bitmap_accept
+ * (cuda/permutation_bitmap.cu:27-48, :105-133) at world = 1. */
+int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint32_t* permutations,
+                               const uint64_t* frontier, uint64_t count, uint32_t* bitmap, uint64_t bitmap_bits,
+                               uint64_t* next, uint64_t* next_count, uint64_t next_capacity, uint32_t* error,
+                               void* stream);
+
+/* ---------------------------------------------------------- versions */
+uint32_t mgbfs_hip_abi_version(void);
+/* Device properties: CU count, wave size, LDS bytes per workgroup, HBM bytes. */
+int mgbfs_hip_device_info(int32_t device, uint32_t* compute_units, uint32_t* wave_size, uint32_t* lds_bytes,
+                          uint64_t* total_bytes, uint64_t* free_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGBFS_HIP_H */

@@ -1,0 +1,221 @@
+"""ctypes binding of libmgbfs_hip.so (include/mgbfs_hip.h).
+
+The native library is mandatory: there is no CPU fallback.  Loading fails
+loudly (NativeUnavailable) when the in-tree build is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libmgbfs_hip.so"
+CLI_PATH = PKG / "bin" / "mgbfs"
+
+MGBFS_LAYER, MGBFS_COMPLETE, MGBFS_RESOURCE = 0, 1, 2
+BACKENDS = {"auto": 0, "permutation_bitmap": 1, "sorted": 2, "segmented": 3, "generic": 4}
+BACKEND_NAMES = {v: k for k, v in BACKENDS.items()}
+
+
+class NativeUnavailable(RuntimeError):
+    """The HIP library is not built or cannot be loaded."""
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("elements", C.c_uint32), ("rows", C.c_uint32), ("cols", C.c_uint32),
+                ("generators", C.c_uint32), ("inverse_closed", C.c_uint32),
+                ("permutations", C.POINTER(C.c_uint32)), ("matrices", C.POINTER(C.c_int64)),
+                ("moduli", C.POINTER(C.c_uint32)), ("start", C.POINTER(C.c_int64))]
+
+
+class EngineOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("backend", C.c_uint32), ("capacity", C.c_uint64),
+                ("vram_budget", C.c_uint64), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("backend", C.c_uint32), ("history_layers", C.c_uint32), ("state_bytes", C.c_uint32),
+                ("depth", C.c_uint32), ("current_count", C.c_uint64), ("total_states", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("peak_layer", C.c_uint64), ("candidates", C.c_uint64),
+                ("unique_candidates", C.c_uint64), ("kernel_seconds", C.c_double),
+                ("kernel_launches", C.c_uint64), ("algorithmic_bytes", C.c_uint64)]
+
+
+_LIB = None
+
+
+def load():
+    """Load the in-tree libmgbfs_hip.so (raises NativeUnavailable)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise NativeUnavailable(f"{LIB_PATH} is not built; run `python -m multigpubfs_amd.build`")
+    try:
+        lib = C.CDLL(str(LIB_PATH))
+    except OSError as exc:
+        raise NativeUnavailable(str(exc)) from exc
+    p, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+    lib.mgbfs_engine_create.argtypes = [C.POINTER(GraphDesc), C.POINTER(EngineOptions), C.POINTER(p),
+                                        C.c_char_p, C.c_size_t]
+    lib.mgbfs_engine_create.restype = C.c_int
+    lib.mgbfs_engine_advance.argtypes = [p, C.POINTER(u64), C.POINTER(u32)]
+    lib.mgbfs_engine_advance.restype = C.c_int
+    lib.mgbfs_engine_run.argtypes = [p, u32, C.c_double]
+    lib.mgbfs_engine_run.restype = C.c_int
+    lib.mgbfs_engine_query.argtypes = [p, C.POINTER(EngineInfo)]
+    lib.mgbfs_engine_query.restype = C.c_int
+    lib.mgbfs_engine_reset.argtypes = [p]
+    lib.mgbfs_engine_reset.restype = C.c_int
+    lib.mgbfs_engine_layer_sizes.argtypes = [p, C.POINTER(u64), u32, C.POINTER(u32)]
+    lib.mgbfs_engine_layer_sizes.restype = C.c_int
+    lib.mgbfs_engine_sample.argtypes = [p, u32, u64, C.POINTER(C.c_int64), C.POINTER(u64)]
+    lib.mgbfs_engine_sample.restype = C.c_int
+    lib.mgbfs_engine_layer_keys.argtypes = [p, C.POINTER(u64), u64, C.POINTER(u64)]
+    lib.mgbfs_engine_layer_keys.restype = C.c_int
+    lib.mgbfs_engine_backend_name.argtypes = [p]
+    lib.mgbfs_engine_backend_name.restype = C.c_char_p
+    lib.mgbfs_engine_destroy.argtypes = [p]
+    lib.mgbfs_engine_destroy.restype = None
+    lib.mgbfs_hip_abi_version.argtypes = []
+    lib.mgbfs_hip_abi_version.restype = u32
+    lib.mgbfs_hip_device_info.argtypes = [i32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32),
+                                          C.POINTER(u64), C.POINTER(u64)]
+    lib.mgbfs_hip_device_info.restype = C.c_int
+    _LIB = lib
+    return lib
+
+
+def _array(ty, values):
+    values = list(values)
+    return (ty * max(1, len(values)))(*values)
+
+
+class NativeEngine:
+    """One exact BFS engine instance on one GPU (mgbfs_engine_* wrapper)."""
+
+    def __init__(self, graph: dict, *, device: int = 0, backend: str | int = "auto", capacity: int = 0,
+                 vram_budget: int = 0, inverse_closed: bool | None = None):
+        from .graph_definition import GraphDefinition
+        lib = load()
+        gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
+        data = gd.to_dict()
+        a = data["action"]
+        self.elements = len(data["start"])
+        self._keep = []
+        desc = GraphDesc()
+        desc.elements = self.elements
+        desc.generators = len(a["generators"])
+        desc.inverse_closed = int(gd.inverse_closed() if inverse_closed is None else inverse_closed)
+        start = _array(C.c_int64, data["start"])
+        desc.start = start
+        self._keep.append(start)
+        if a["kind"] == "permutation":
+            desc.kind, desc.rows, desc.cols = 0, self.elements, 1
+            perms = _array(C.c_uint32, [v for g in a["generators"] for v in g])
+            desc.permutations = perms
+            self._keep.append(perms)
+        else:
+            desc.kind, desc.rows, desc.cols = 1, a["rows"], a["cols"]
+            mats = _array(C.c_int64, [v for g in a["generators"] for v in g["matrix"]])
+            mods = _array(C.c_uint32, [g["modulo"] for g in a["generators"]])
+            desc.matrices, desc.moduli = mats, mods
+            self._keep += [mats, mods]
+        opts = EngineOptions()
+        opts.device = device
+        opts.backend = BACKENDS[backend] if isinstance(backend, str) else int(backend)
+        opts.capacity = capacity
+        opts.vram_budget = vram_budget
+        handle = C.c_void_p()
+        err = C.create_string_buffer(256)
+        rc = lib.mgbfs_engine_create(C.byref(desc), C.byref(opts), C.byref(handle), err, 256)
+        if rc:
+            raise RuntimeError(err.value.decode() or f"ENGINE_CREATE_{rc}")
+        self._lib, self._h = lib, handle
+        self.fatal = 0
+        self.status = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mgbfs_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def advance(self) -> int:
+        nc, fatal = C.c_uint64(), C.c_uint32()
+        rc = self._lib.mgbfs_engine_advance(self._h, C.byref(nc), C.byref(fatal))
+        if rc < 0:
+            raise RuntimeError(f"ENGINE_ADVANCE_{rc}: {self._lib.mgbfs_engine_backend_name(None).decode()}")
+        self.fatal = fatal.value
+        self.status = rc
+        return rc
+
+    def run(self, max_layers: int = 0, max_seconds: float = 0.0) -> int:
+        rc = self._lib.mgbfs_engine_run(self._h, max_layers, max_seconds)
+        if rc < 0:
+            raise RuntimeError(f"ENGINE_RUN_{rc}: {self._lib.mgbfs_engine_backend_name(None).decode()}")
+        self.status = rc
+        return rc
+
+    def reset(self):
+        rc = self._lib.mgbfs_engine_reset(self._h)
+        if rc:
+            raise RuntimeError(f"ENGINE_RESET_{rc}")
+        self.status = None
+        self.fatal = 0
+
+    def layer_sizes(self) -> list[int]:
+        n = C.c_uint32()
+        self._lib.mgbfs_engine_layer_sizes(self._h, None, 0, C.byref(n))
+        buf = (C.c_uint64 * max(1, n.value))()
+        self._lib.mgbfs_engine_layer_sizes(self._h, buf, n.value, C.byref(n))
+        return list(buf[:n.value])
+
+    def info(self) -> dict:
+        info = EngineInfo()
+        self._lib.mgbfs_engine_query(self._h, C.byref(info))
+        out = {name: getattr(info, name) for name, _ in EngineInfo._fields_}
+        out["backend_name"] = self.backend_name
+        return out
+
+    @property
+    def backend_name(self) -> str:
+        return self._lib.mgbfs_engine_backend_name(self._h).decode()
+
+    def layer_keys(self) -> list[int]:
+        n = C.c_uint64()
+        cur = self.info()["current_count"]
+        buf = (C.c_uint64 * max(1, cur))()
+        rc = self._lib.mgbfs_engine_layer_keys(self._h, buf, cur, C.byref(n))
+        if rc:
+            raise RuntimeError(f"ENGINE_LAYER_KEYS_{rc}")
+        return list(buf[:n.value])
+
+    def sample(self, which: int = 0, limit: int = 1000) -> list[list[int]]:
+        rows = (C.c_int64 * max(1, limit * self.elements))()
+        n = C.c_uint64()
+        rc = self._lib.mgbfs_engine_sample(self._h, which, limit, rows, C.byref(n))
+        if rc:
+            raise RuntimeError(f"ENGINE_SAMPLE_{rc}")
+        w = self.elements
+        flat = rows[:n.value * w]
+        return [list(flat[i * w:(i + 1) * w]) for i in range(n.value)]
+
+
+def device_info(device: int = 0) -> dict:
+    lib = load()
+    cu, ws, lds = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    tot, fr = C.c_uint64(), C.c_uint64()
+    rc = lib.mgbfs_hip_device_info(device, C.byref(cu), C.byref(ws), C.byref(lds), C.byref(tot), C.byref(fr))
+    if rc:
+        raise RuntimeError(f"HIP_DEVICE_INFO_{rc}")
+    return dict(compute_units=cu.value, wave_size=ws.value, lds_bytes=lds.value, total_bytes=tot.value,
+                free_bytes=fr.value)

@@ -1,0 +1,151 @@
+// Permutation-bitmap backend: fused expand -> Lehmer rank -> bit claim ->
+// wave/workgroup-aggregated append.  One pass over the frontier per layer.
+//
+// Replaces the reference's bitmap_route + bitmap_accept pair
+// (cuda/permutation_bitmap.cu:27-48, :105-133) and the fused prototype
+// expand_kernel (gpu/cayley_bfs.cu:84-124) with an MI355X-first kernel:
+//  * frontier keys are read fully coalesced (tile-major: item*256 + tid),
+//  * the generator table is a by-value kernel argument (scalar registers),
+//  * a plain load of the bitmap word filters already-visited children before
+//    any atomic (bits only go 0 -> 1, so a stale read can only under-report),
+//  * new children are appended with one global atomic per workgroup.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mgbfs {
+
+constexpr int kBitmapThreads = 256;
+constexpr int kChildrenPerThread = 24;
+
+template <int G>
+__global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
+    const uint64_t* __restrict__ frontier, uint64_t count, PackedGenerators gens,
+    uint32_t* __restrict__ bitmap, uint64_t bitmap_bits, uint64_t* __restrict__ next,
+    unsigned long long* __restrict__ next_count, uint64_t next_cap, uint32_t* __restrict__ error) {
+  constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
+  __shared__ uint64_t parents[kItems][kBitmapThreads];
+  __shared__ uint32_t wave_sums[kBitmapThreads / kWave];
+  __shared__ unsigned long long wg_base;
+  const uint32_t n = gens.degree;
+  const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kBitmapThreads * kItems;
+  uint32_t newmask = 0;
+  uint32_t errbits = 0;
+
+  // Phase 1: claim.  Children are not kept; the few new ones are recomputed
+  // from the LDS copy of the parent when they are written.
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t p = tile + static_cast<uint64_t>(it) * kBitmapThreads + threadIdx.x;
+    const bool valid = p < count;
+    const uint64_t key = valid ? frontier[p] : 0;
+    parents[it][threadIdx.x] = key;
+    if (!valid) continue;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint64_t c = apply_packed(key, gens.src[g], n);
+      bool bad;
+      const uint64_t r = lehmer_rank_packed(c, n, &bad);
+      if (bad || r >= bitmap_bits) {
+        errbits |= kErrDomain;
+        continue;
+      }
+      uint32_t* word = bitmap + (r >> 5);
+      const uint32_t bit = 1u << (r & 31u);
+      uint32_t seen = *word;
+      if (!(seen & bit)) {
+        seen = __hip_atomic_fetch_or(word, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(seen & bit)) newmask |= 1u << (it * G + g);
+      }
+    }
+  }
+
+  // Phase 2: workgroup exclusive scan of per-thread new-child counts and one
+  // global reservation per workgroup.
+  const uint32_t mine = __popc(newmask);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t incl = mine;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, kWave);
+    if (lane >= static_cast<uint32_t>(off)) incl += v;
+  }
+  if (lane == kWave - 1) wave_sums[wave] = incl;
+  __syncthreads();
+  uint32_t wave_off = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBitmapThreads / kWave; ++w) {
+    const uint32_t s = wave_sums[w];
+    if (w < static_cast<int>(wave)) wave_off += s;
+    total += s;
+  }
+  if (threadIdx.x == 0) wg_base = total ? atomicAdd(next_count, static_cast<unsigned long long>(total)) : 0ull;
+  __syncthreads();
+
+  // Phase 3: write this thread's new children contiguously.
+  if (mine) {
+    uint64_t pos = wg_base + wave_off + (incl - mine);
+    if (pos + mine > next_cap) {
+      errbits |= kErrFutureCap;
+    } else {
+      uint32_t m = newmask;
+      while (m) {
+        const uint32_t b = __ffs(m) - 1;
+        m &= m - 1;
+        const uint32_t it = b / G, g = b - it * G;
+        next[pos++] = apply_packed(parents[it][threadIdx.x], gens.src[g], n);
+      }
+    }
+  }
+  if (errbits) atomicOr(error, errbits);
+}
+
+__global__ void bitmap_seed_kernel(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits,
+                                   uint64_t* frontier, uint32_t* error) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  bool bad;
+  const uint64_t r = lehmer_rank_packed(start_key, degree, &bad);
+  if (bad || r >= bitmap_bits) {
+    atomicOr(error, kErrDomain);
+    return;
+  }
+  atomicOr(bitmap + (r >> 5), 1u << (r & 31u));
+  frontier[0] = start_key;
+}
+
+template <int G>
+static hipError_t launch_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens,
+                                uint32_t* bitmap, uint64_t bits, uint64_t* next, unsigned long long* next_count,
+                                uint64_t next_cap, uint32_t* error, hipStream_t stream) {
+  constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
+  const uint64_t per_block = static_cast<uint64_t>(kBitmapThreads) * kItems;
+  const uint64_t blocks = (count + per_block - 1) / per_block;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bitmap_expand_kernel<G>, dim3(static_cast<uint32_t>(blocks)), dim3(kBitmapThreads), 0, stream,
+                     frontier, count, gens, bitmap, bits, next, next_count, next_cap, error);
+  return hipGetLastError();
+}
+
+hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens, uint32_t* bitmap,
+                         uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t next_cap,
+                         uint32_t* error, hipStream_t stream) {
+  switch (gens.count) {
+#define MGBFS_G(k) \
+  case k:          \
+    return launch_expand<k>(frontier, count, gens, bitmap, bitmap_bits, next, next_count, next_cap, error, stream);
+    MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
+    MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
+#undef MGBFS_G
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits, uint64_t* frontier,
+                       uint32_t* error, hipStream_t stream) {
+  hipLaunchKernelGGL(bitmap_seed_kernel, dim3(1), dim3(64), 0, stream, degree, start_key, bitmap, bitmap_bits,
+                     frontier, error);
+  return hipGetLastError();
+}
+
+}  // namespace mgbfs

@@ -1,0 +1,116 @@
+// Shared device/host helpers for the MI355X exact BFS engine (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mgbfs {
+
+constexpr int kWave = 64;
+constexpr uint32_t kMaxDegree = 16;     // nibble-packed permutations: n <= 16
+constexpr uint32_t kMaxGenerators = 16; // packed engine generator table size
+
+// Sticky device error word bits (caller-zeroed, OR-accumulated).
+constexpr uint32_t kErrTableFull = 1u;      // open-addressing table has no free slot
+constexpr uint32_t kErrVisitedCap = 2u;     // visited/row arena capacity exceeded
+constexpr uint32_t kErrFutureCap = 4u;      // next-frontier capacity exceeded
+constexpr uint32_t kErrBadRef = 8u;         // invalid row reference
+constexpr uint32_t kErrCount = 32u;         // received count above bound
+constexpr uint32_t kErrBucketCap = 64u;     // sorted engine: bucket above LDS capacity
+constexpr uint32_t kErrDomain = 128u;       // state outside the packed/rank domain
+
+// Generator table passed by value: wave-uniform indices become scalar loads.
+struct PackedGenerators {
+  uint32_t count;   // generators
+  uint32_t degree;  // n
+  uint8_t src[kMaxGenerators][kMaxDegree];  // child[i] = parent[src[g][i]]
+};
+
+__host__ __device__ inline uint32_t nib(uint64_t key, uint32_t i) {
+  return static_cast<uint32_t>((key >> (4u * i)) & 15u);
+}
+
+// child[i] = parent[src[i]] on nibble-packed keys, done on 32-bit halves.
+__device__ __forceinline__ uint64_t apply_packed(uint64_t key, const uint8_t* src, uint32_t n) {
+  const uint32_t lo = static_cast<uint32_t>(key);
+  const uint32_t hi = static_cast<uint32_t>(key >> 32);
+  uint32_t clo = 0, chi = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kMaxDegree; ++i) {
+    if (i < n) {
+      const uint32_t s = src[i];
+      const uint32_t word = s < 8 ? lo : hi;
+      const uint32_t v = (word >> (4u * (s & 7u))) & 15u;
+      if (i < 8) clo |= v << (4u * i);
+      else chi |= v << (4u * (i - 8));
+    }
+  }
+  return (static_cast<uint64_t>(chi) << 32) | clo;
+}
+
+__host__ inline uint64_t apply_packed_host(uint64_t key, const uint8_t* src, uint32_t n) {
+  uint64_t child = 0;
+  for (uint32_t i = 0; i < n; ++i) child |= ((key >> (4u * src[i])) & 15u) << (4u * i);
+  return child;
+}
+
+// Lexicographic rank (position 0 most significant) of a nibble-packed
+// permutation of 0..n-1.  Digit i = symbol - #smaller symbols already used.
+// Sets *bad when the key is not a permutation of 0..n-1.
+__host__ __device__ __forceinline__ uint64_t lehmer_rank_packed(uint64_t key, uint32_t n, bool* bad) {
+  uint32_t seen = 0;
+  uint64_t r = 0;
+  bool invalid = false;
+#pragma unroll
+  for (uint32_t i = 0; i < kMaxDegree; ++i) {
+    if (i < n) {
+      const uint32_t s = nib(key, i);
+      const uint32_t below = seen & ((1u << s) - 1u);
+#if defined(__HIP_DEVICE_COMPILE__)
+      const uint32_t d = s - static_cast<uint32_t>(__popc(below));
+#else
+      const uint32_t d = s - static_cast<uint32_t>(__builtin_popcount(below));
+#endif
+      invalid |= (s >= n) | ((seen >> s) & 1u);
+      seen |= 1u << s;
+      r = r * (n - i) + d;
+    }
+  }
+  *bad = invalid;
+  return r;
+}
+
+// Bijective 64-bit mixer (murmur3 fmix64 constants) and its inverse; the
+// sorted engine orders states by mix(key) so buckets are balanced.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__host__ __device__ constexpr uint64_t inverse_odd(uint64_t a) {
+  uint64_t x = a;  // Newton iteration for a^-1 mod 2^64 (a odd)
+  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+  return x;
+}
+
+__host__ __device__ __forceinline__ uint64_t unmix64(uint64_t x) {
+  constexpr uint64_t i1 = inverse_odd(0xff51afd7ed558ccdull);
+  constexpr uint64_t i2 = inverse_odd(0xc4ceb9fe1a85ec53ull);
+  x ^= x >> 33;
+  x *= i2;
+  x ^= x >> 33;
+  x *= i1;
+  x ^= x >> 33;
+  return x;
+}
+
+__host__ inline uint64_t factorial_u64(uint32_t n) {
+  uint64_t f = 1;
+  for (uint32_t i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+
+}  // namespace mgbfs

@@ -1,0 +1,472 @@
+// Host runtime: graph intake, backend selection, the bitmap engine and the
+// C ABI of mgbfs_engine_*.
+//
+// Layer loop contract (crates/mgbfs-runtime/src/generic_native.rs:106-126):
+// every advance() enqueues the layer's kernels on the engine stream and reads
+// back one small control block (next count + sticky error word) per layer.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace mgbfs {
+
+bool Graph::packable() const {
+  if (kind != 0 || elements == 0 || elements > kMaxDegree || generators == 0 || generators > kMaxGenerators)
+    return false;
+  for (int64_t v : start)
+    if (v < 0 || v > 15) return false;
+  return true;
+}
+
+bool Graph::distinct_perm() const {
+  if (kind != 0 || elements > kMaxDegree) return false;
+  uint32_t seen = 0;
+  for (int64_t v : start) {
+    if (v < 0 || v >= static_cast<int64_t>(elements) || ((seen >> v) & 1u)) return false;
+    seen |= 1u << v;
+  }
+  return true;
+}
+
+uint64_t Graph::start_key() const {
+  uint64_t key = 0;
+  for (uint32_t i = 0; i < elements; ++i) key |= (static_cast<uint64_t>(start[i]) & 15u) << (4u * i);
+  return key;
+}
+
+PackedGenerators Graph::packed_generators() const {
+  PackedGenerators pg;
+  std::memset(&pg, 0, sizeof(pg));
+  pg.count = generators;
+  pg.degree = elements;
+  for (uint32_t g = 0; g < generators; ++g)
+    for (uint32_t i = 0; i < elements; ++i) pg.src[g][i] = static_cast<uint8_t>(perms[g * elements + i]);
+  return pg;
+}
+
+void Engine::fill_info(mgbfs_engine_info& info) const {
+  std::memset(&info, 0, sizeof(info));
+  info.backend = backend;
+  info.history_layers = history_layers;
+  info.state_bytes = state_bytes;
+  info.depth = layers.empty() ? 0 : static_cast<uint32_t>(layers.size() - 1);
+  info.current_count = layers.empty() ? 0 : layers.back();
+  for (uint64_t s : layers) {
+    info.total_states += s;
+    info.peak_layer = std::max(info.peak_layer, s);
+  }
+  info.device_bytes = device_bytes;
+  info.candidates = candidates;
+  info.unique_candidates = unique_candidates;
+  info.kernel_seconds = kernel_seconds;
+  info.kernel_launches = launches;
+  info.algorithmic_bytes = streamed_bytes;
+}
+
+static uint64_t vram_budget(const mgbfs_engine_options& o) {
+  if (o.vram_budget) return o.vram_budget;
+  size_t fr = 0, tot = 0;
+  MGBFS_HIP(hipMemGetInfo(&fr, &tot));
+  const uint64_t reserve = 2ull << 30;  // generic_memory.rs:9-19 cold reserve
+  return fr > reserve ? fr - reserve : 0;
+}
+
+static void unpack_rows(const std::vector<uint64_t>& keys, uint32_t n, std::vector<int64_t>& rows) {
+  rows.resize(keys.size() * n);
+  for (size_t k = 0; k < keys.size(); ++k)
+    for (uint32_t i = 0; i < n; ++i) rows[k * n + i] = static_cast<int64_t>((keys[k] >> (4u * i)) & 15u);
+}
+
+// ------------------------------------------------------------ bitmap engine
+
+class BitmapEngine final : public Engine {
+ public:
+  BitmapEngine(const Graph& g, const mgbfs_engine_options& o) : gens_(g.packed_generators()), n_(g.elements) {
+    device_ = o.device;
+    backend = MGBFS_BACKEND_BITMAP;
+    history_layers = g.inverse_closed ? 3 : 0;
+    MGBFS_HIP(hipSetDevice(device_));
+    MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    MGBFS_HIP(hipEventCreate(&ev0_));
+    MGBFS_HIP(hipEventCreate(&ev1_));
+    budget_ = vram_budget(o);
+    domain_ = factorial_u64(n_);
+    const uint64_t words = (domain_ + 31) / 32;
+    const uint64_t bitmap_bytes = words * 4;
+    if (bitmap_bytes + 64 * 1024 > budget_) throw std::runtime_error("BITMAP_VRAM_CAPACITY");
+    bitmap_.allocate(bitmap_bytes);
+    control_.allocate(64);
+    MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), 64, hipHostMallocDefault));
+    cap_limit_ = o.capacity ? o.capacity : domain_;
+    // Frontier buffers grow on demand up to what the budget admits.
+    const uint64_t frontier_budget = (budget_ - bitmap_bytes) / 2 / 8;
+    cap_limit_ = std::min(cap_limit_, frontier_budget);
+    if (cap_limit_ == 0) throw std::runtime_error("BITMAP_VRAM_CAPACITY");
+    grow(front_, front_cap_, std::min<uint64_t>(cap_limit_, 1u << 16));
+    grow(next_, next_cap_, std::min<uint64_t>(cap_limit_, 1u << 16));
+    start_key_ = g.start_key();
+    reset();
+  }
+
+  void reset() override {
+    MGBFS_HIP(hipSetDevice(device_));
+    MGBFS_HIP(fill_u32(bitmap_.as<uint32_t>(), 0u, (domain_ + 31) / 32, stream_));
+    MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 8, stream_));
+    MGBFS_HIP(bitmap_seed(n_, start_key_, bitmap_.as<uint32_t>(), domain_, front_.as<uint64_t>(),
+                          error_word(), stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    read_control();
+    if (host_ctl_[1]) throw std::runtime_error("BITMAP_START_DOMAIN");
+    count_ = 1;
+    prev_count_ = 0;
+    layers.assign(1, 1);
+    candidates = unique_candidates = 0;
+    kernel_seconds = 0.0;
+    launches = 0;
+    streamed_bytes = 0;
+    terminal = false;
+    update_bytes();
+  }
+  ~BitmapEngine() override {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (host_ctl_) (void)hipHostFree(host_ctl_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int advance(uint64_t* next_count, uint32_t* fatal) override {
+    if (terminal) throw std::runtime_error("ENGINE_TERMINAL");
+    MGBFS_HIP(hipSetDevice(device_));
+    const uint64_t remaining = domain_ - total();
+    const uint64_t want = std::min(gens_.count * count_, remaining);
+    if (want > next_cap_) grow_next(std::min(cap_limit_, std::max(want, next_cap_ * 2)));
+    MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 1, stream_));
+    // Chunk the frontier only when the next buffer cannot hold the worst case.
+    const uint64_t chunk = want <= next_cap_ ? count_ : std::max<uint64_t>(1, next_cap_ / gens_.count);
+    MGBFS_HIP(hipEventRecord(ev0_, stream_));
+    uint64_t produced = 0;
+    for (uint64_t begin = 0; begin < count_; begin += chunk) {
+      const uint64_t c = std::min(chunk, count_ - begin);
+      if (chunk != count_) {
+        MGBFS_HIP(hipStreamSynchronize(stream_));
+        read_control();
+        produced = host_ctl_[0];
+        if (produced + c * gens_.count > next_cap_) {
+          *fatal = kErrFutureCap;
+          terminal = true;
+          return MGBFS_RESOURCE;
+        }
+      }
+      ++launches;
+      MGBFS_HIP(bitmap_expand(front_.as<uint64_t>() + begin, c, gens_, bitmap_.as<uint32_t>(), domain_,
+                              next_.as<uint64_t>(), reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()),
+                              next_cap_, error_word(), stream_));
+    }
+    MGBFS_HIP(hipEventRecord(ev1_, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    read_control();
+    float ms = 0.f;
+    MGBFS_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kernel_seconds += ms * 1e-3;
+    candidates += count_ * gens_.count;
+    streamed_bytes += 8 * count_;
+    const uint32_t err = static_cast<uint32_t>(host_ctl_[1]);
+    if (err) {
+      *fatal = err;
+      terminal = true;
+      return MGBFS_RESOURCE;
+    }
+    const uint64_t produced_all = host_ctl_[0];
+    *next_count = produced_all;
+    if (produced_all == 0) {
+      terminal = true;
+      return MGBFS_COMPLETE;
+    }
+    std::swap(front_, next_);
+    std::swap(front_cap_, next_cap_);
+    prev_count_ = count_;
+    count_ = produced_all;
+    layers.push_back(count_);
+    streamed_bytes += 8 * count_;
+    update_bytes();
+    return MGBFS_LAYER;
+  }
+
+  void layer_keys(std::vector<uint64_t>& out) override { copy_keys(front_, count_, count_, out); }
+
+  void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) override {
+    std::vector<uint64_t> keys;
+    if (which == 0) {
+      copy_keys(front_, count_, limit, keys);
+    } else if (prev_count_ && layers.size() >= 2) {
+      // After a swap the previous layer stays intact in next_ until the next advance.
+      copy_keys(next_, prev_count_, limit, keys);
+    }
+    unpack_rows(keys, n_, rows);
+    count = keys.size();
+  }
+  const char* name() const override { return "PERMUTATION_BITMAP_LEHMER"; }
+
+ private:
+  uint32_t* error_word() { return reinterpret_cast<uint32_t*>(control_.as<uint64_t>() + 1); }
+  uint64_t total() const {
+    uint64_t t = 0;
+    for (uint64_t s : layers) t += s;
+    return t;
+  }
+  void read_control() {
+    MGBFS_HIP(hipMemcpyAsync(host_ctl_, control_.as<uint64_t>(), 16, hipMemcpyDeviceToHost, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    host_ctl_[1] &= 0xffffffffull;
+  }
+  void grow(DeviceBuffer& b, uint64_t& cap, uint64_t want) {
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    b.allocate(want * 8);
+    cap = want;
+  }
+  // Grows next_ (which holds the previous layer between advances) keeping
+  // the first rows that the previous-layer sample can still ask for.
+  void grow_next(uint64_t want) {
+    DeviceBuffer fresh;
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    fresh.allocate(want * 8);
+    const uint64_t keep = std::min<uint64_t>(prev_count_, 1000);
+    if (keep)
+      MGBFS_HIP(hipMemcpyAsync(fresh.as<uint64_t>(), next_.as<uint64_t>(), keep * 8, hipMemcpyDeviceToDevice, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    next_ = std::move(fresh);
+    next_cap_ = want;
+  }
+  void copy_keys(const DeviceBuffer& b, uint64_t available, uint64_t limit, std::vector<uint64_t>& out) {
+    const uint64_t c = std::min(available, limit);
+    out.resize(c);
+    if (c) {
+      MGBFS_HIP(hipMemcpyAsync(out.data(), b.as<uint64_t>(), c * 8, hipMemcpyDeviceToHost, stream_));
+      MGBFS_HIP(hipStreamSynchronize(stream_));
+    }
+  }
+  void update_bytes() { device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes(); }
+
+  PackedGenerators gens_;
+  uint32_t n_;
+  uint64_t domain_ = 0, budget_ = 0, cap_limit_ = 0;
+  DeviceBuffer bitmap_, front_, next_, control_;
+  uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, start_key_ = 0;
+  uint64_t* host_ctl_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+Engine* make_bitmap_engine(const Graph& g, const mgbfs_engine_options& o) { return new BitmapEngine(g, o); }
+
+// ----------------------------------------------------------- graph intake
+
+static Graph intake(const mgbfs_graph_desc* d) {
+  if (!d || !d->start || d->elements == 0 || d->generators == 0) throw std::runtime_error("GRAPH_SCHEMA");
+  Graph g;
+  g.kind = d->kind;
+  g.elements = d->elements;
+  g.rows = d->rows;
+  g.cols = d->cols;
+  g.generators = d->generators;
+  g.inverse_closed = d->inverse_closed != 0;
+  g.start.assign(d->start, d->start + d->elements);
+  if (d->kind == 0) {
+    if (!d->permutations) throw std::runtime_error("GRAPH_PERMUTATION_GENERATOR");
+    g.perms.assign(d->permutations, d->permutations + static_cast<size_t>(d->generators) * d->elements);
+    for (uint32_t k = 0; k < d->generators; ++k) {
+      std::vector<uint8_t> used(d->elements, 0);
+      for (uint32_t i = 0; i < d->elements; ++i) {
+        const uint32_t v = g.perms[k * d->elements + i];
+        if (v >= d->elements || used[v]) throw std::runtime_error("GRAPH_PERMUTATION_GENERATOR");
+        used[v] = 1;
+      }
+    }
+  } else if (d->kind == 1) {
+    if (!d->matrices || !d->moduli || d->rows == 0 || d->cols == 0 ||
+        static_cast<uint64_t>(d->rows) * d->cols != d->elements)
+      throw std::runtime_error("GRAPH_MATRIX_SHAPE");
+    g.matrices.assign(d->matrices, d->matrices + static_cast<size_t>(d->generators) * d->rows * d->rows);
+    g.moduli.assign(d->moduli, d->moduli + d->generators);
+    for (uint32_t m : g.moduli)
+      if (m == 1 || m > (1u << 31)) throw std::runtime_error("GRAPH_MATRIX_MODULUS");
+  } else {
+    throw std::runtime_error("GRAPH_ACTION_KIND");
+  }
+  return g;
+}
+
+static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
+  uint32_t backend = o.backend;
+  if (backend == MGBFS_BACKEND_AUTO) {
+    if (g.packable() && g.distinct_perm()) backend = MGBFS_BACKEND_BITMAP;
+    else throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
+  }
+  switch (backend) {
+    case MGBFS_BACKEND_BITMAP:
+      if (!g.packable() || !g.distinct_perm()) throw std::runtime_error("BITMAP_UNSUPPORTED_GRAPH");
+      return make_bitmap_engine(g, o);
+    default:
+      throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
+  }
+}
+
+}  // namespace mgbfs
+
+using mgbfs::Engine;
+
+static void put_error(char* text, size_t cap, const char* what) {
+  if (text && cap) {
+    std::strncpy(text, what, cap - 1);
+    text[cap - 1] = 0;
+  }
+}
+
+static thread_local std::string g_last_error;
+
+#define MGBFS_GUARD(...)                                    \
+  try {                                                     \
+    __VA_ARGS__                                             \
+  } catch (const mgbfs::HipError& e) {                      \
+    g_last_error = e.what();                                \
+    return -static_cast<int>(e.code == 0 ? 1 : e.code);     \
+  } catch (const std::exception& e) {                       \
+    g_last_error = e.what();                                \
+    return -1;                                              \
+  }
+
+extern "C" int mgbfs_engine_create(const mgbfs_graph_desc* graph, const mgbfs_engine_options* options,
+                                   void** engine, char* error_text, size_t error_capacity) {
+  if (!engine || !options) return hipErrorInvalidValue;
+  *engine = nullptr;
+  try {
+    mgbfs::Graph g = mgbfs::intake(graph);
+    *engine = mgbfs::create(g, *options);
+    return 0;
+  } catch (const std::exception& e) {
+    put_error(error_text, error_capacity, e.what());
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+extern "C" int mgbfs_engine_advance(void* engine, uint64_t* next_count, uint32_t* fatal) {
+  if (!engine) return hipErrorInvalidValue;
+  uint64_t nc = 0;
+  uint32_t f = 0;
+  MGBFS_GUARD({
+    const int s = static_cast<Engine*>(engine)->advance(&nc, &f);
+    if (next_count) *next_count = nc;
+    if (fatal) *fatal = f;
+    return s;
+  })
+}
+
+extern "C" int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_seconds) {
+  if (!engine) return hipErrorInvalidValue;
+  MGBFS_GUARD({
+    Engine* e = static_cast<Engine*>(engine);
+    const auto t0 = std::chrono::steady_clock::now();
+    int s = MGBFS_LAYER;
+    uint32_t done = 0;
+    while (!e->terminal) {
+      if (max_layers && done >= max_layers) break;
+      if (max_seconds > 0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= max_seconds)
+        break;
+      uint64_t nc;
+      uint32_t f;
+      s = e->advance(&nc, &f);
+      ++done;
+    }
+    return s;
+  })
+}
+
+extern "C" int mgbfs_engine_query(void* engine, mgbfs_engine_info* info) {
+  if (!engine || !info) return hipErrorInvalidValue;
+  static_cast<Engine*>(engine)->fill_info(*info);
+  return 0;
+}
+
+extern "C" int mgbfs_engine_reset(void* engine) {
+  if (!engine) return hipErrorInvalidValue;
+  MGBFS_GUARD({
+    static_cast<Engine*>(engine)->reset();
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_engine_layer_sizes(void* engine, uint64_t* sizes, uint32_t capacity, uint32_t* count) {
+  if (!engine || !count) return hipErrorInvalidValue;
+  const auto& l = static_cast<Engine*>(engine)->layers;
+  *count = static_cast<uint32_t>(l.size());
+  if (sizes)
+    for (uint32_t i = 0; i < capacity && i < l.size(); ++i) sizes[i] = l[i];
+  return 0;
+}
+
+extern "C" int mgbfs_engine_sample(void* engine, uint32_t which, uint64_t limit, int64_t* rows, uint64_t* count) {
+  if (!engine || !count) return hipErrorInvalidValue;
+  MGBFS_GUARD({
+    std::vector<int64_t> r;
+    uint64_t c = 0;
+    static_cast<Engine*>(engine)->sample(which, limit, r, c);
+    if (rows && !r.empty()) std::memcpy(rows, r.data(), r.size() * sizeof(int64_t));
+    *count = c;
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_engine_layer_keys(void* engine, uint64_t* keys, uint64_t capacity, uint64_t* count) {
+  if (!engine || !count) return hipErrorInvalidValue;
+  MGBFS_GUARD({
+    std::vector<uint64_t> k;
+    static_cast<Engine*>(engine)->layer_keys(k);
+    *count = k.size();
+    if (k.size() > capacity) return static_cast<int>(hipErrorInvalidValue);
+    if (keys && !k.empty()) std::memcpy(keys, k.data(), k.size() * 8);
+    return 0;
+  })
+}
+
+extern "C" const char* mgbfs_engine_backend_name(void* engine) {
+  return engine ? static_cast<Engine*>(engine)->name() : g_last_error.c_str();
+}
+
+extern "C" void mgbfs_engine_destroy(void* engine) { delete static_cast<Engine*>(engine); }
+
+// ---------------------------------------------------- kernel-level ABI
+
+extern "C" int mgbfs_packed_bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits,
+                                        uint64_t* frontier, uint32_t* error, void* stream) {
+  if (degree < 1 || degree > mgbfs::kMaxDegree || !bitmap || !frontier || !error) return hipErrorInvalidValue;
+  return mgbfs::bitmap_seed(degree, start_key, bitmap, bitmap_bits, frontier, error,
+                            static_cast<hipStream_t>(stream));
+}
+
+extern "C" int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint32_t* permutations,
+                                          const uint64_t* frontier, uint64_t count, uint32_t* bitmap,
+                                          uint64_t bitmap_bits, uint64_t* next, uint64_t* next_count,
+                                          uint64_t next_capacity, uint32_t* error, void* stream) {
+  if (degree < 1 || degree > mgbfs::kMaxDegree || generators < 1 || generators > mgbfs::kMaxGenerators ||
+      !permutations || !bitmap || !next || !next_count || !error || (count && !frontier))
+    return hipErrorInvalidValue;
+  mgbfs::PackedGenerators pg;
+  std::memset(&pg, 0, sizeof(pg));
+  pg.count = generators;
+  pg.degree = degree;
+  for (uint32_t g = 0; g < generators; ++g)
+    for (uint32_t i = 0; i < degree; ++i) {
+      const uint32_t v = permutations[g * degree + i];
+      if (v >= degree) return hipErrorInvalidValue;
+      pg.src[g][i] = static_cast<uint8_t>(v);
+    }
+  return mgbfs::bitmap_expand(frontier, count, pg, bitmap, bitmap_bits, next,
+                              reinterpret_cast<unsigned long long*>(next_count), next_capacity, error,
+                              static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,97 @@
+// Host runtime of the single-GPU exact BFS path (C++ side of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "common.hpp"
+#include "../../include/mgbfs_hip.h"
+
+namespace mgbfs {
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+#define MGBFS_HIP(call)                                                                          \
+  do {                                                                                           \
+    hipError_t _e = (call);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+      throw ::mgbfs::HipError(_e, std::string(#call) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+// Owned device allocation (freed on destruction, on the owning device).
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept { swap(o); }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept { swap(o); return *this; }
+  ~DeviceBuffer() { release(); }
+  void allocate(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 8;
+    MGBFS_HIP(hipMalloc(&ptr_, bytes));
+    bytes_ = bytes;
+  }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  void swap(DeviceBuffer& o) noexcept {
+    std::swap(ptr_, o.ptr_);
+    std::swap(bytes_, o.bytes_);
+  }
+  template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Host copy of the graph description (validated).
+struct Graph {
+  uint32_t kind = 0, elements = 0, rows = 0, cols = 0, generators = 0;
+  bool inverse_closed = false;
+  std::vector<uint32_t> perms;    // [G][elements]
+  std::vector<int64_t> matrices;  // [G][rows*rows]
+  std::vector<uint32_t> moduli;   // [G]
+  std::vector<int64_t> start;     // [elements]
+
+  bool packable() const;      // permutation, n <= 16, all start values in 0..15
+  bool distinct_perm() const; // start is a permutation of 0..n-1
+  uint64_t start_key() const;
+  PackedGenerators packed_generators() const;
+};
+
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  virtual int advance(uint64_t* next_count, uint32_t* fatal) = 0;
+  virtual void reset() = 0;
+  virtual void layer_keys(std::vector<uint64_t>& out) = 0;
+  virtual void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) = 0;
+  virtual const char* name() const = 0;
+  virtual void fill_info(mgbfs_engine_info& info) const;
+
+  std::vector<uint64_t> layers;  // layer sizes, depth 0 first
+  uint64_t candidates = 0, unique_candidates = 0, device_bytes = 0;
+  double kernel_seconds = 0.0;
+  uint64_t launches = 0;        // hot-kernel launches
+  uint64_t streamed_bytes = 0;  // algorithmic HBM bytes of the hot kernels
+  uint32_t backend = 0, history_layers = 0, state_bytes = 8;
+  bool terminal = false;
+
+ protected:
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+};
+
+Engine* make_bitmap_engine(const Graph& g, const mgbfs_engine_options& o);
+
+}  // namespace mgbfs
