@@ -32,7 +32,7 @@ def cpu_baseline(n: int, seconds_target: float) -> dict:
     the edge budget is reached)."""
     from oracle import oracle as O
     cores = min(os.cpu_count() or 1, 16)
-    budget = int(seconds_target * 40e6)  # ~40 M edges/s on 16 cores
+    budget = int(seconds_target * 150e6)  # ~150 M edges/s on 16 cores
     t0 = time.perf_counter()
     r = O.packed_bfs(n, threads=cores, stop_after_edges=budget)
     dt = time.perf_counter() - t0
@@ -47,9 +47,11 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=13)
-    ap.add_argument("--backend", default="permutation_bitmap")
+    ap.add_argument("--backend", default="sorted")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -57,6 +59,14 @@ def main(argv=None):
                                    "multi-GPU sharding is not part of this build", "n_gpus": args.gpus}))
         return 2
 
+    if args.calibrate:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1073741824 bytes read + written", file=sys.stderr, flush=True)
+        del src, dst
     from multigpubfs_amd import NativeEngine, lrx
     golden = json.loads((ROOT / "tests" / "golden" / "lrx_layers.json").read_text())["layers"].get(str(args.n))
     n = args.n
@@ -79,6 +89,8 @@ def main(argv=None):
     kernel_s = 0.0
     launches = 0
     algo_bytes = 0
+    phase_s = [0.0] * 4
+    phase_b = [0] * 4
     t0 = time.perf_counter()
     for _ in range(args.steps):
         engine.reset()
@@ -87,15 +99,32 @@ def main(argv=None):
         kernel_s += info["kernel_seconds"]
         launches += info["kernel_launches"]
         algo_bytes += info["algorithmic_bytes"]
+        for i, (ps, pb) in enumerate(engine.phases(4)):
+            phase_s[i] += ps
+            phase_b[i] += pb
         sizes = engine.layer_sizes()
         exact &= status == 1 and ((sizes == golden) if golden else sum(sizes) == states)
     sync()
     elapsed = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     value = args.steps * edges / elapsed
-    avg_launch_s = kernel_s / max(1, launches)
-    bytes_per_launch = algo_bytes / max(1, launches)
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    layers = len(sizes)
+    if info["backend"] == 1:
+        names = ["bitmap_expand_kernel"]
+        models = ["8F frontier read + 8N next-layer write (bitmap probes excluded)"]
+    else:
+        names = ["expand_count_kernel+expand_scatter_kernel", "partition_count_kernel+partition_scatter_kernel",
+                 "unique_diff_kernel", "compact_kernel"]
+        models = ["16F + 8C", "24C", "8C + 8F + 8F_prev + 8N", "16N"]
+    phases = []
+    for i, name in enumerate(names):
+        if phase_s[i] <= 0:
+            continue
+        gbs = phase_b[i] / phase_s[i] / 1e9
+        phases.append({"kernel": name, "ms_per_step": 1e3 * phase_s[i] / args.steps,
+                       "bytes_per_step": phase_b[i] // args.steps, "achieved_gbs": gbs, "bytes_model": models[i]})
+    dom = max(phases, key=lambda p: p["ms_per_step"])
+    per_launch_layers = layers * args.steps
     result = {
         "metric": f"expanded edges/s on S{n} LRX BFS",
         "value": value,
@@ -110,15 +139,20 @@ def main(argv=None):
         "dtype": "u64",
         "data": "synthetic: implicit Cayley graph (LRX generators on S_n from the identity), no dataset",
         "config": {"workload": f"S{n} LRX full exact BFS, one GPU", "backend": engine.backend_name,
-                   "states": states, "layers": len(sizes), "edges_per_step": edges,
+                   "states": states, "layers": layers, "edges_per_step": edges,
                    "peak_layer": max(sizes), "layer_table_exact": bool(exact),
+                   "unique_candidates_per_step": info["unique_candidates"],
+                   "generated_candidates_per_step": info["candidates"],
                    "device_bytes": info["device_bytes"],
                    "baseline": "reference 8x H200, 11.716894 s (docs/validation/lrx13-eight-h200.md:26)"},
-        "roofline": {"bound": "hbm", "kernel": "bitmap_expand_kernel", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launches_per_step": launches // max(1, args.steps),
-                     "avg_launch_ms": 1e3 * avg_launch_s, "bytes_per_launch": bytes_per_launch,
-                     "bytes_model": "8F frontier read + 8N next-layer write per layer (bitmap probes excluded)"},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "avg_launch_ms": dom["ms_per_step"] / layers,
+                     "bytes_per_launch": dom["bytes_per_step"] / layers,
+                     "bytes_model": dom["bytes_model"],
+                     "all_hot_kernels_gbs": algo_bytes / kernel_s / 1e9 if kernel_s else 0.0,
+                     "phases": phases},
     }
     engine.close()
     if not args.no_cpu_baseline:

@@ -48,6 +48,10 @@ extern "C" {
 #define MGBFS_BACKEND_SEGMENTED 3 /* sort-unique + segmented open-addressing hash */
 #define MGBFS_BACKEND_GENERIC 4  /* SoA int64/u8 states, all-visited exact hash   */
 
+/* Option flags. */
+#define MGBFS_FLAG_TEST_COARSE_BUCKETS 1u /* sorted backend: start 3 bucket bits coarse
+                                             so large layers take the overflow-retry path */
+
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are
  * HOST pointers, consumed before mgbfs_engine_create returns. */
@@ -70,7 +74,7 @@ typedef struct mgbfs_engine_options {
   uint32_t backend;      /* MGBFS_BACKEND_*                                    */
   uint64_t capacity;     /* max states per layer (0 = plan from free VRAM)     */
   uint64_t vram_budget;  /* bytes the engine may allocate (0 = free - reserve) */
-  uint32_t flags;        /* reserved, 0                                        */
+  uint32_t flags;        /* MGBFS_FLAG_* (0 for normal runs)                   */
   uint32_t reserved;
 } mgbfs_engine_options;
 
@@ -88,6 +92,7 @@ typedef struct mgbfs_engine_info {
   double kernel_seconds;   /* summed device time of the hot kernels          */
   uint64_t kernel_launches;   /* launches of the hot kernels                 */
   uint64_t algorithmic_bytes; /* modelled streaming HBM bytes of those kernels */
+  uint64_t retries;        /* sorted backend: layer passes redone at finer buckets */
 } mgbfs_engine_info;
 
 /* ------------------------------------------------------------ engine */
@@ -105,6 +110,11 @@ int mgbfs_engine_advance(void* engine, uint64_t* next_count, uint32_t* fatal);
  * max_seconds (0 = unlimited).  Returns the last advance() status. */
 int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_seconds);
 int mgbfs_engine_query(void* engine, mgbfs_engine_info* info);
+/* Per-phase device seconds and modelled streaming HBM bytes of the hot
+ * kernels, summed over the layers run so far.  Sorted backend phases:
+ * 0 expand+route, 1 partition, 2 unique+difference, 3 compaction.
+ * Bitmap backend: phase 0 is the fused expand kernel. */
+int mgbfs_engine_phases(void* engine, double* seconds, uint64_t* bytes, uint32_t count);
 /* Clears the visited state and re-seeds the start state, keeping every
  * allocation (repeat runs of the same graph, e.g. timing loops). */
 int mgbfs_engine_reset(void* engine);

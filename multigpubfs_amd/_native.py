@@ -38,7 +38,7 @@ class EngineInfo(C.Structure):
                 ("depth", C.c_uint32), ("current_count", C.c_uint64), ("total_states", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("peak_layer", C.c_uint64), ("candidates", C.c_uint64),
                 ("unique_candidates", C.c_uint64), ("kernel_seconds", C.c_double),
-                ("kernel_launches", C.c_uint64), ("algorithmic_bytes", C.c_uint64)]
+                ("kernel_launches", C.c_uint64), ("algorithmic_bytes", C.c_uint64), ("retries", C.c_uint64)]
 
 
 _LIB = None
@@ -65,6 +65,8 @@ def load():
     lib.mgbfs_engine_run.restype = C.c_int
     lib.mgbfs_engine_query.argtypes = [p, C.POINTER(EngineInfo)]
     lib.mgbfs_engine_query.restype = C.c_int
+    lib.mgbfs_engine_phases.argtypes = [p, C.POINTER(C.c_double), C.POINTER(u64), u32]
+    lib.mgbfs_engine_phases.restype = C.c_int
     lib.mgbfs_engine_reset.argtypes = [p]
     lib.mgbfs_engine_reset.restype = C.c_int
     lib.mgbfs_engine_layer_sizes.argtypes = [p, C.POINTER(u64), u32, C.POINTER(u32)]
@@ -95,7 +97,7 @@ class NativeEngine:
     """One exact BFS engine instance on one GPU (mgbfs_engine_* wrapper)."""
 
     def __init__(self, graph: dict, *, device: int = 0, backend: str | int = "auto", capacity: int = 0,
-                 vram_budget: int = 0, inverse_closed: bool | None = None):
+                 vram_budget: int = 0, inverse_closed: bool | None = None, flags: int = 0):
         from .graph_definition import GraphDefinition
         lib = load()
         gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
@@ -126,6 +128,7 @@ class NativeEngine:
         opts.backend = BACKENDS[backend] if isinstance(backend, str) else int(backend)
         opts.capacity = capacity
         opts.vram_budget = vram_budget
+        opts.flags = flags
         handle = C.c_void_p()
         err = C.create_string_buffer(256)
         rc = lib.mgbfs_engine_create(C.byref(desc), C.byref(opts), C.byref(handle), err, 256)
@@ -185,6 +188,12 @@ class NativeEngine:
         out = {name: getattr(info, name) for name, _ in EngineInfo._fields_}
         out["backend_name"] = self.backend_name
         return out
+
+    def phases(self, count: int = 4) -> list[tuple[float, int]]:
+        secs = (C.c_double * count)()
+        byts = (C.c_uint64 * count)()
+        self._lib.mgbfs_engine_phases(self._h, secs, byts, count)
+        return [(secs[i], byts[i]) for i in range(count)]
 
     @property
     def backend_name(self) -> str:

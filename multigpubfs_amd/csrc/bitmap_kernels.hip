@@ -39,8 +39,10 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
     const uint64_t key = valid ? frontier[p] : 0;
     parents[it][threadIdx.x] = key;
     if (!valid) continue;
+    const uint32_t skip = gens.tag_shift < 64 ? static_cast<uint32_t>(key >> gens.tag_shift) : 0u;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
+      if (static_cast<uint32_t>(g) + 1 == skip) continue;
       const uint64_t c = apply_packed(key, gens.src[g], n);
       bool bad;
       const uint64_t r = lehmer_rank_packed(c, n, &bad);
@@ -92,7 +94,9 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
         const uint32_t b = __ffs(m) - 1;
         m &= m - 1;
         const uint32_t it = b / G, g = b - it * G;
-        next[pos++] = apply_packed(parents[it][threadIdx.x], gens.src[g], n);
+        uint64_t c = apply_packed(parents[it][threadIdx.x], gens.src[g], n);
+        if (gens.tag_shift < 64) c |= static_cast<uint64_t>(gens.child_tag[g]) << gens.tag_shift;
+        next[pos++] = c;
       }
     }
   }

@@ -19,10 +19,26 @@ constexpr uint32_t kErrBucketCap = 64u;     // sorted engine: bucket above LDS c
 constexpr uint32_t kErrDomain = 128u;       // state outside the packed/rank domain
 
 // Generator table passed by value: wave-uniform indices become scalar loads.
+//
+// Parent-edge pruning: a stored state may carry a small tag t = 1 + index of
+// the inverse of the generator that produced it (0 = no tag).  Expanding that
+// state skips generator t - 1: its child is the recorded parent, which lies
+// in the previous layer and is already visited.  Any one parent edge is a
+// valid tag, so the choice among duplicates does not matter.
 struct PackedGenerators {
   uint32_t count;   // generators
   uint32_t degree;  // n
   uint8_t src[kMaxGenerators][kMaxDegree];  // child[i] = parent[src[g][i]]
+  uint8_t child_tag[kMaxGenerators];        // tag written into a child made by g
+  uint32_t tag_shift;                        // bitmap backend: tag bit position (64 = off)
+};
+
+// Bijective mixer on B-bit keys (B = 4n): xorshift by B/2+1 (self-inverse
+// within B bits) and odd multipliers mod 2^B.  The sorted backend stores
+// (mix(key) << (64 - B)) | tag, so the top bits are uniform bucket ids.
+struct KeyMix {
+  uint32_t bits, shift;
+  uint64_t mask, m1, m2, i1, i2;
 };
 
 __host__ __device__ inline uint32_t nib(uint64_t key, uint32_t i) {
@@ -105,6 +121,36 @@ __host__ __device__ __forceinline__ uint64_t unmix64(uint64_t x) {
   x *= i1;
   x ^= x >> 33;
   return x;
+}
+
+__host__ __device__ __forceinline__ uint64_t keymix(uint64_t x, const KeyMix& k) {
+  x ^= x >> k.shift;
+  x = (x * k.m1) & k.mask;
+  x ^= x >> k.shift;
+  x = (x * k.m2) & k.mask;
+  x ^= x >> k.shift;
+  return x;
+}
+
+__host__ __device__ __forceinline__ uint64_t keyunmix(uint64_t x, const KeyMix& k) {
+  x ^= x >> k.shift;
+  x = (x * k.i2) & k.mask;
+  x ^= x >> k.shift;
+  x = (x * k.i1) & k.mask;
+  x ^= x >> k.shift;
+  return x;
+}
+
+__host__ inline KeyMix make_keymix(uint32_t bits) {
+  KeyMix k;
+  k.bits = bits;
+  k.shift = bits / 2 + 1;
+  k.mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+  k.m1 = 0xff51afd7ed558ccdull & k.mask;
+  k.m2 = 0xc4ceb9fe1a85ec53ull & k.mask;
+  k.i1 = inverse_odd(0xff51afd7ed558ccdull) & k.mask;
+  k.i2 = inverse_odd(0xc4ceb9fe1a85ec53ull) & k.mask;
+  return k;
 }
 
 __host__ inline uint64_t factorial_u64(uint32_t n) {

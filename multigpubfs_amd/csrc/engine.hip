@@ -42,9 +42,21 @@ PackedGenerators Graph::packed_generators() const {
   std::memset(&pg, 0, sizeof(pg));
   pg.count = generators;
   pg.degree = elements;
+  pg.tag_shift = 64;
   for (uint32_t g = 0; g < generators; ++g)
     for (uint32_t i = 0; i < elements; ++i) pg.src[g][i] = static_cast<uint8_t>(perms[g * elements + i]);
   return pg;
+}
+
+std::vector<int> inverse_index(const PackedGenerators& gens) {
+  std::vector<int> inv(gens.count, -1);
+  for (uint32_t a = 0; a < gens.count; ++a)
+    for (uint32_t b = 0; b < gens.count && inv[a] < 0; ++b) {
+      bool ok = true;
+      for (uint32_t i = 0; i < gens.degree && ok; ++i) ok = gens.src[b][gens.src[a][i]] == i;
+      if (ok) inv[a] = static_cast<int>(b);
+    }
+  return inv;
 }
 
 void Engine::fill_info(mgbfs_engine_info& info) const {
@@ -64,6 +76,7 @@ void Engine::fill_info(mgbfs_engine_info& info) const {
   info.kernel_seconds = kernel_seconds;
   info.kernel_launches = launches;
   info.algorithmic_bytes = streamed_bytes;
+  info.retries = retry_count();
 }
 
 static uint64_t vram_budget(const mgbfs_engine_options& o) {
@@ -86,6 +99,13 @@ class BitmapEngine final : public Engine {
  public:
   BitmapEngine(const Graph& g, const mgbfs_engine_options& o) : gens_(g.packed_generators()), n_(g.elements) {
     device_ = o.device;
+    // Parent-edge tags live in nibble 15 when the state uses at most 15.
+    if (n_ <= 15 && gens_.count <= 14) {
+      const std::vector<int> inv = inverse_index(gens_);
+      gens_.tag_shift = 60;
+      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_tag[a] = inv[a] >= 0 ? static_cast<uint8_t>(inv[a] + 1) : 0;
+    }
+    key_mask_ = n_ >= 16 ? ~0ull : ((1ull << (4 * n_)) - 1);
     backend = MGBFS_BACKEND_BITMAP;
     history_layers = g.inverse_closed ? 3 : 0;
     MGBFS_HIP(hipSetDevice(device_));
@@ -248,6 +268,7 @@ class BitmapEngine final : public Engine {
     if (c) {
       MGBFS_HIP(hipMemcpyAsync(out.data(), b.as<uint64_t>(), c * 8, hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipStreamSynchronize(stream_));
+      for (auto& k : out) k &= key_mask_;
     }
   }
   void update_bytes() { device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes(); }
@@ -256,7 +277,7 @@ class BitmapEngine final : public Engine {
   uint32_t n_;
   uint64_t domain_ = 0, budget_ = 0, cap_limit_ = 0;
   DeviceBuffer bitmap_, front_, next_, control_;
-  uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, start_key_ = 0;
+  uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, start_key_ = 0, key_mask_ = ~0ull;
   uint64_t* host_ctl_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };
@@ -303,13 +324,17 @@ static Graph intake(const mgbfs_graph_desc* d) {
 static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
   uint32_t backend = o.backend;
   if (backend == MGBFS_BACKEND_AUTO) {
-    if (g.packable() && g.distinct_perm()) backend = MGBFS_BACKEND_BITMAP;
+    if (g.packable() && g.inverse_closed) backend = MGBFS_BACKEND_SORTED;
+    else if (g.packable() && g.distinct_perm()) backend = MGBFS_BACKEND_BITMAP;
     else throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
   }
   switch (backend) {
     case MGBFS_BACKEND_BITMAP:
       if (!g.packable() || !g.distinct_perm()) throw std::runtime_error("BITMAP_UNSUPPORTED_GRAPH");
       return make_bitmap_engine(g, o);
+    case MGBFS_BACKEND_SORTED:
+      if (!g.packable()) throw std::runtime_error("SORTED_UNSUPPORTED_GRAPH");
+      return make_sorted_engine(g, o);
     default:
       throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
   }
@@ -390,6 +415,12 @@ extern "C" int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_se
 extern "C" int mgbfs_engine_query(void* engine, mgbfs_engine_info* info) {
   if (!engine || !info) return hipErrorInvalidValue;
   static_cast<Engine*>(engine)->fill_info(*info);
+  return 0;
+}
+
+extern "C" int mgbfs_engine_phases(void* engine, double* seconds, uint64_t* bytes, uint32_t count) {
+  if (!engine || !seconds || !bytes) return hipErrorInvalidValue;
+  static_cast<Engine*>(engine)->fill_phases(seconds, bytes, count);
   return 0;
 }
 

@@ -77,7 +77,15 @@ class Engine {
   virtual void layer_keys(std::vector<uint64_t>& out) = 0;
   virtual void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) = 0;
   virtual const char* name() const = 0;
+  virtual uint64_t retry_count() const { return 0; }
   virtual void fill_info(mgbfs_engine_info& info) const;
+  // Per-phase device seconds and modelled streaming bytes (backend-defined).
+  virtual void fill_phases(double* seconds, uint64_t* bytes, uint32_t n) const {
+    for (uint32_t i = 0; i < n; ++i) {
+      seconds[i] = i == 0 ? kernel_seconds : 0.0;
+      bytes[i] = i == 0 ? streamed_bytes : 0;
+    }
+  }
 
   std::vector<uint64_t> layers;  // layer sizes, depth 0 first
   uint64_t candidates = 0, unique_candidates = 0, device_bytes = 0;
@@ -93,5 +101,8 @@ class Engine {
 };
 
 Engine* make_bitmap_engine(const Graph& g, const mgbfs_engine_options& o);
+Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o);
+// Index of each generator's inverse in the set (-1 when absent).
+std::vector<int> inverse_index(const PackedGenerators& gens);
 
 }  // namespace mgbfs

@@ -11,6 +11,39 @@ hipError_t bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, ui
 hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens, uint32_t* bitmap,
                          uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t next_cap,
                          uint32_t* error, hipStream_t stream);
+// Sorted-bank backend limits.
+constexpr uint32_t kMaxPassBits = 10;     // buckets per routing pass (LDS counters)
+constexpr uint32_t kMaxBucketBits = 20;   // sub-buckets per layer: 2^20
+constexpr uint32_t kMaxFineBits = 10;     // bank partition bits beyond the layer's K
+constexpr uint32_t kUniqueSlots = 8192;   // LDS hash slots per sub-bucket
+constexpr uint32_t kUniqueCap = 4096;     // candidates per sub-bucket (load factor <= 0.5)
+
+// Stored-value layout of the sorted backend: (keymix(key) << tagbits) | tag.
+struct SortedLayout {
+  KeyMix km;
+  uint32_t tagbits;  // 64 - 4n
+  uint32_t prune;    // parent-edge tags in use
+  uint64_t tagmask;  // (1 << tagbits) - 1
+};
+
+uint64_t scan_workspace_words(uint64_t n);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* workspace,
+                              hipStream_t stream);
+hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
+                       hipStream_t stream);
+hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
+                               uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan, uint32_t* scan_ws,
+                               uint64_t* out, hipStream_t stream);
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* scan1, uint32_t NB1, uint32_t K1, uint32_t K2,
+                            uint32_t NC, uint32_t* counts, uint32_t* scan2, uint32_t* scan_ws, uint64_t* out,
+                            hipStream_t stream);
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, const uint64_t* cur,
+                              const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
+                              uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
+                              unsigned long long* unique_total, uint32_t tagbits, hipStream_t stream);
+hipError_t sorted_compact(const uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, uint32_t Ko,
+                          const uint32_t* off, uint64_t* bank, hipStream_t stream);
+
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);
 hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 

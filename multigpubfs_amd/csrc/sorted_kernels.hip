@@ -1,0 +1,458 @@
+// Sort-unique backend with three rotating VRAM banks (inverse-closed sets).
+//
+// States are stored as h = mix64(key) (a bijection, so equality of h is
+// full-key equality).  A bank holds one BFS layer, partitioned by the top
+// Ko bits of h ("fine buckets"), with an offset table of 2^Ko + 1 entries.
+// One layer d -> d+1:
+//   expand_count / expand_scatter : children of bank[d], bucketed by the top
+//        K1 bits into owner lanes (LDS counting sort per 4096-element tile,
+//        contiguous run writes).                              8F + 8F + 8C
+//   partition_count / _scatter    : each lane split by the next K2 bits.
+//                                                             8C + 16C
+//   unique_diff                   : one workgroup per sub-bucket (K = K1+K2
+//        bits): LDS open-addressing set of the candidates (in-layer dedup),
+//        probe with the same sub-bucket of bank[d] and bank[d-1] (exact
+//        three-bank retention for inverse-closed generators), survivors
+//        regrouped by Ko fine bits and written back in place. 8C + 8F + 8F' + 8N
+//   compact                       : survivors -> bank[d+1].   16N
+// Replaces the reference's sorted-run owner (cuda/segmented_hash_batch.cu:
+// 157-215: merge sort by (hash,state), binary-search probe of bank runs,
+// exclusive-scan compaction) and its batch owner routing
+// (cuda/segmented_batch_route.cu:52-151) with LDS-resident bucket work.
+#include <algorithm>
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mgbfs {
+
+constexpr int kRouteThreads = 256;
+constexpr int kTile = 4096;
+constexpr int kTilePerThread = kTile / kRouteThreads;
+constexpr int kUniqueThreads = 256;
+
+__device__ __forceinline__ uint32_t top_bits(uint64_t h, uint32_t bits) {
+  return bits ? static_cast<uint32_t>(h >> (64u - bits)) : 0u;
+}
+
+__device__ __forceinline__ uint32_t mid_bits(uint64_t h, uint32_t skip, uint32_t bits) {
+  // bits [64-skip-bits, 64-skip) of h
+  return bits ? static_cast<uint32_t>((h << skip) >> (64u - bits)) : 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(v, off, kWave);
+    if (lane >= static_cast<uint32_t>(off)) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of `count` (<= 4 * blockDim) LDS counters into `out`.
+__device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* wave_sums) {
+  const uint32_t per = (count + blockDim.x - 1) / blockDim.x;  // <= 4
+  const uint32_t base = threadIdx.x * per;
+  uint32_t v[4];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = base + k;
+    v[k] = (k < static_cast<int>(per) && i < count) ? in[i] : 0u;
+    s += v[k];
+  }
+  const uint32_t incl = wave_incl_scan(s);
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == kWave - 1) wave_sums[wave] = incl;
+  __syncthreads();
+  uint32_t run = incl - s;
+  for (uint32_t w = 0; w < wave; ++w) run += wave_sums[w];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = base + k;
+    if (k < static_cast<int>(per) && i < count) out[i] = run;
+    run += v[k];
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ expansion
+
+// Stored value of a child made by generator g from `key`.
+__device__ __forceinline__ uint64_t child_value(uint64_t key, const PackedGenerators& gens, int g,
+                                                const SortedLayout& L) {
+  const uint64_t child = apply_packed(key, gens.src[g], gens.degree);
+  return (keymix(child, L.km) << L.tagbits) | gens.child_tag[g];
+}
+
+__device__ __forceinline__ uint32_t parent_skip(uint64_t v, const SortedLayout& L) {
+  // generator index + 1 to skip (0: none)
+  return L.prune ? static_cast<uint32_t>(v & L.tagmask) : 0u;
+}
+
+template <int G>
+__global__ __launch_bounds__(kRouteThreads) void expand_count_kernel(const uint64_t* __restrict__ bank, uint64_t F,
+                                                                      PackedGenerators gens, SortedLayout L,
+                                                                      uint32_t K1, uint32_t NB,
+                                                                      uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[1u << kMaxPassBits];
+  const uint32_t buckets = 1u << K1;
+  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const uint64_t p0 = F * blockIdx.x / NB, p1 = F * (blockIdx.x + 1) / NB;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const uint64_t v = bank[p];
+    const uint64_t key = keyunmix(v >> L.tagbits, L.km);
+    const uint32_t skip = parent_skip(v, L);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (static_cast<uint32_t>(g) + 1 != skip) atomicAdd(&hist[top_bits(child_value(key, gens, g, L), K1)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) counts[static_cast<uint64_t>(i) * NB + blockIdx.x] = hist[i];
+}
+
+// Shared tile machinery: A[0..ne) holds the tile's elements; each is routed
+// to bucket bkt(h); writes go to out[running[b] + rank within tile].
+struct TileLds {
+  uint64_t a[kTile];
+  uint64_t s[kTile];
+  uint32_t lhist[1u << kMaxPassBits];
+  uint32_t lstart[1u << kMaxPassBits];
+  uint32_t running[1u << kMaxPassBits];
+  uint32_t wave_sums[kRouteThreads / kWave];
+  uint32_t fill;
+};
+
+template <typename BucketFn>
+__device__ __forceinline__ void route_tile(TileLds& t, uint32_t ne, uint32_t buckets, BucketFn bkt,
+                                           uint64_t* __restrict__ out) {
+  uint32_t rank[kTilePerThread];
+#pragma unroll
+  for (int k = 0; k < kTilePerThread; ++k) {
+    const uint32_t i = k * kRouteThreads + threadIdx.x;
+    rank[k] = i < ne ? atomicAdd(&t.lhist[bkt(t.a[i])], 1u) : 0u;
+  }
+  __syncthreads();
+  lds_exclusive_scan(t.lhist, t.lstart, buckets, t.wave_sums);
+#pragma unroll
+  for (int k = 0; k < kTilePerThread; ++k) {
+    const uint32_t i = k * kRouteThreads + threadIdx.x;
+    if (i < ne) {
+      const uint64_t h = t.a[i];
+      t.s[t.lstart[bkt(h)] + rank[k]] = h;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
+    const uint64_t h = t.s[j];
+    const uint32_t b = bkt(h);
+    out[static_cast<uint64_t>(t.running[b]) + (j - t.lstart[b])] = h;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) {
+    t.running[i] += t.lhist[i];
+    t.lhist[i] = 0;
+  }
+  __syncthreads();
+}
+
+template <int G>
+__global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uint64_t* __restrict__ bank, uint64_t F,
+                                                                        PackedGenerators gens, SortedLayout L,
+                                                                        uint32_t K1, uint32_t NB,
+                                                                        const uint32_t* __restrict__ offsets,
+                                                                        uint64_t* __restrict__ out) {
+  __shared__ TileLds t;
+  const uint32_t buckets = 1u << K1;
+  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) {
+    t.running[i] = offsets[static_cast<uint64_t>(i) * NB + blockIdx.x];
+    t.lhist[i] = 0;
+  }
+  const uint64_t p0 = F * blockIdx.x / NB, p1 = F * (blockIdx.x + 1) / NB;
+  constexpr uint32_t kParents = kTile / G;
+  auto bkt = [K1](uint64_t h) { return top_bits(h, K1); };
+  for (uint64_t t0 = p0; t0 < p1; t0 += kParents) {
+    if (threadIdx.x == 0) t.fill = 0;
+    __syncthreads();
+    const uint32_t np = static_cast<uint32_t>(p1 - t0 < kParents ? p1 - t0 : kParents);
+    for (uint32_t q = threadIdx.x; q < np; q += kRouteThreads) {
+      const uint64_t v = bank[t0 + q];
+      const uint64_t key = keyunmix(v >> L.tagbits, L.km);
+      const uint32_t skip = parent_skip(v, L);
+      uint32_t at = atomicAdd(&t.fill, skip ? G - 1u : static_cast<uint32_t>(G));
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (static_cast<uint32_t>(g) + 1 != skip) t.a[at++] = child_value(key, gens, g, L);
+    }
+    __syncthreads();
+    route_tile(t, t.fill, buckets, bkt, out);
+  }
+}
+
+// ------------------------------------------------------------ partition
+
+__global__ __launch_bounds__(kRouteThreads) void partition_count_kernel(const uint64_t* __restrict__ in,
+                                                                         const uint32_t* __restrict__ scan1, uint32_t NB1,
+                                                                         uint32_t K1, uint32_t K2, uint32_t NC,
+                                                                         uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[1u << kMaxPassBits];
+  const uint32_t buckets = 1u << K2;
+  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const uint32_t j = blockIdx.x / NC, c = blockIdx.x % NC;
+  const uint64_t gs = scan1[static_cast<uint64_t>(j) * NB1], ge = scan1[static_cast<uint64_t>(j + 1) * NB1];
+  const uint64_t r0 = gs + (ge - gs) * c / NC, r1 = gs + (ge - gs) * (c + 1) / NC;
+  for (uint64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) atomicAdd(&hist[mid_bits(in[i], K1, K2)], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < buckets; b += blockDim.x)
+    counts[((static_cast<uint64_t>(j) << K2) + b) * NC + c] = hist[b];
+}
+
+__global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const uint64_t* __restrict__ in,
+                                                                           const uint32_t* __restrict__ scan1,
+                                                                           uint32_t NB1, uint32_t K1, uint32_t K2,
+                                                                           uint32_t NC,
+                                                                           const uint32_t* __restrict__ offsets,
+                                                                           uint64_t* __restrict__ out) {
+  __shared__ TileLds t;
+  const uint32_t buckets = 1u << K2;
+  const uint32_t j = blockIdx.x / NC, c = blockIdx.x % NC;
+  for (uint32_t b = threadIdx.x; b < buckets; b += kRouteThreads) {
+    t.running[b] = offsets[((static_cast<uint64_t>(j) << K2) + b) * NC + c];
+    t.lhist[b] = 0;
+  }
+  __syncthreads();
+  const uint64_t gs = scan1[static_cast<uint64_t>(j) * NB1], ge = scan1[static_cast<uint64_t>(j + 1) * NB1];
+  const uint64_t r0 = gs + (ge - gs) * c / NC, r1 = gs + (ge - gs) * (c + 1) / NC;
+  auto bkt = [K1, K2](uint64_t h) { return mid_bits(h, K1, K2); };
+  for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
+    const uint32_t ne = static_cast<uint32_t>(r1 - t0 < kTile ? r1 - t0 : kTile);
+    for (uint32_t i = threadIdx.x; i < ne; i += kRouteThreads) t.a[i] = in[t0 + i];
+    __syncthreads();
+    route_tile(t, ne, buckets, bkt, out);
+  }
+}
+
+// ------------------------------------------------------ unique + difference
+
+__device__ __forceinline__ bool same_state(uint64_t a, uint64_t b, uint32_t tagbits) {
+  return ((a ^ b) >> tagbits) == 0;
+}
+
+struct UniqueLds {
+  uint64_t table[kUniqueSlots];
+  uint32_t found[kUniqueSlots / 32];
+  uint32_t fhist[1u << kMaxFineBits];
+  uint32_t fstart[1u << kMaxFineBits];
+  uint32_t wave_sums[kUniqueThreads / kWave];
+  uint32_t flags;  // bit0: candidate ~0 present, bit1: ~0 found in a bank
+  uint32_t unique;
+};
+
+constexpr uint64_t kEmpty = ~0ull;
+
+__device__ __forceinline__ int lds_find(const UniqueLds& u, uint64_t h, uint32_t tagbits) {
+  uint32_t slot = static_cast<uint32_t>(h >> tagbits) & (kUniqueSlots - 1);
+  for (uint32_t probe = 0; probe < kUniqueSlots; ++probe) {
+    const uint64_t v = u.table[slot];
+    if (v == kEmpty) return -1;
+    if (same_state(v, h, tagbits)) return static_cast<int>(slot);
+    slot = (slot + 1) & (kUniqueSlots - 1);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void probe_bank(UniqueLds& u, const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
+                                           uint32_t tagbits) {
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kUniqueThreads) {
+    const uint64_t h = bank[i];
+    if (h == kEmpty) {
+      atomicOr(&u.flags, 2u);
+      continue;
+    }
+    const int s = lds_find(u, h, tagbits);
+    if (s >= 0) atomicOr(&u.found[s >> 5], 1u << (s & 31));
+  }
+}
+
+// grid = 2^K blocks; block f handles sub-bucket f.
+__global__ __launch_bounds__(kUniqueThreads) void unique_diff_kernel(
+    uint64_t* __restrict__ cand, const uint32_t* __restrict__ cscan, uint32_t NC, uint32_t K,
+    const uint64_t* __restrict__ cur, const uint32_t* __restrict__ cur_off, uint32_t Kc,
+    const uint64_t* __restrict__ prev, const uint32_t* __restrict__ prev_off, uint32_t Kp, uint32_t Ko,
+    uint32_t* __restrict__ fine_counts, uint32_t* __restrict__ error, unsigned long long* __restrict__ unique_total,
+    uint32_t tagbits) {
+  __shared__ UniqueLds u;
+  const uint32_t f = blockIdx.x;
+  const uint64_t s = cscan[static_cast<uint64_t>(f) * NC], e = cscan[static_cast<uint64_t>(f + 1) * NC];
+  const uint32_t fine_bits = Ko - K, fine = 1u << fine_bits;
+  const uint64_t c = e - s;
+  if (c > kUniqueCap) {
+    if (threadIdx.x == 0) atomicOr(error, kErrBucketCap);
+    for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads)
+      fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = 0;
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < kUniqueSlots; i += kUniqueThreads) u.table[i] = kEmpty;
+  for (uint32_t i = threadIdx.x; i < kUniqueSlots / 32; i += kUniqueThreads) u.found[i] = 0;
+  for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
+  if (threadIdx.x == 0) {
+    u.flags = 0;
+    u.unique = 0;
+  }
+  __syncthreads();
+  // In-layer dedup: insert every candidate (full 64-bit equality).
+  uint32_t mine = 0;
+  for (uint64_t i = s + threadIdx.x; i < e; i += kUniqueThreads) {
+    const uint64_t h = cand[i];
+    if (h == kEmpty) {
+      if (!(atomicOr(&u.flags, 1u) & 1u)) ++mine;
+      continue;
+    }
+    uint32_t slot = static_cast<uint32_t>(h >> tagbits) & (kUniqueSlots - 1);
+    while (true) {
+      const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
+      if (old == kEmpty) {
+        ++mine;
+        break;
+      }
+      if (same_state(old, h, tagbits)) break;
+      slot = (slot + 1) & (kUniqueSlots - 1);
+    }
+  }
+  if (mine) atomicAdd(&u.unique, mine);
+  __syncthreads();
+  // Difference against the two retained layers (same sub-bucket only).
+  if (cur) {
+    const uint32_t sh = Kc - K;
+    probe_bank(u, cur, cur_off[static_cast<uint64_t>(f) << sh], cur_off[static_cast<uint64_t>(f + 1) << sh], tagbits);
+  }
+  if (prev) {
+    const uint32_t sh = Kp - K;
+    probe_bank(u, prev, prev_off[static_cast<uint64_t>(f) << sh], prev_off[static_cast<uint64_t>(f + 1) << sh],
+               tagbits);
+  }
+  __syncthreads();
+  const bool max_survives = (u.flags & 3u) == 1u;
+  // Survivors grouped by Ko fine bits.
+  for (uint32_t slot = threadIdx.x; slot < kUniqueSlots; slot += kUniqueThreads) {
+    const uint64_t h = u.table[slot];
+    if (h != kEmpty && !((u.found[slot >> 5] >> (slot & 31)) & 1u)) atomicAdd(&u.fhist[mid_bits(h, K, fine_bits)], 1u);
+  }
+  if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
+  __syncthreads();
+  lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
+  for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads)
+    fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = u.fhist[i];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = u.fstart[i];
+  __syncthreads();
+  // All candidates are in LDS: write survivors back in place at cand[s..).
+  for (uint32_t slot = threadIdx.x; slot < kUniqueSlots; slot += kUniqueThreads) {
+    const uint64_t h = u.table[slot];
+    if (h != kEmpty && !((u.found[slot >> 5] >> (slot & 31)) & 1u)) {
+      const uint32_t pos = atomicAdd(&u.fhist[mid_bits(h, K, fine_bits)], 1u);
+      cand[s + pos] = h;
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (max_survives) cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
+    atomicAdd(unique_total, static_cast<unsigned long long>(u.unique));
+  }
+}
+
+// grid = 2^K blocks: survivors of sub-bucket f -> bank rows.
+__global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand,
+                                                                const uint32_t* __restrict__ cscan, uint32_t NC,
+                                                                uint32_t K, uint32_t Ko,
+                                                                const uint32_t* __restrict__ off,
+                                                                uint64_t* __restrict__ bank) {
+  const uint32_t f = blockIdx.x;
+  const uint64_t s = cscan[static_cast<uint64_t>(f) * NC];
+  const uint32_t sh = Ko - K;
+  const uint64_t lo = off[static_cast<uint64_t>(f) << sh], hi = off[static_cast<uint64_t>(f + 1) << sh];
+  for (uint64_t i = threadIdx.x; i < hi - lo; i += kRouteThreads) bank[lo + i] = cand[s + i];
+}
+
+// Start layer: bank[0] = mix(start); offsets at Ko bits.
+__global__ void sorted_seed_kernel(uint64_t start_key, SortedLayout L, uint64_t* bank, uint32_t* off, uint32_t Ko) {
+  const uint64_t h = keymix(start_key, L.km) << L.tagbits;
+  const uint32_t b = top_bits(h, Ko);
+  const uint64_t n = (1ull << Ko) + 1;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    off[i] = i <= b ? 0u : 1u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) bank[0] = h;
+}
+
+// ------------------------------------------------------------ launchers
+
+hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
+                       hipStream_t stream) {
+  const uint64_t n = (1ull << Ko) + 1;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(sorted_seed_kernel, dim3(blocks), dim3(256), 0, stream, start_key, L, bank, off, Ko);
+  return hipGetLastError();
+}
+
+template <int G>
+static hipError_t expand_route_g(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
+                                 uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan, uint32_t* scan_ws,
+                                 uint64_t* out, hipStream_t stream) {
+  hipLaunchKernelGGL(expand_count_kernel<G>, dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L, K1, NB,
+                     counts);
+  hipError_t e = exclusive_scan_u32(counts, (static_cast<uint64_t>(1) << K1) * NB, scan, scan_ws, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(expand_scatter_kernel<G>, dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L, K1, NB,
+                     scan, out);
+  return hipGetLastError();
+}
+
+hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
+                               uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan, uint32_t* scan_ws,
+                               uint64_t* out, hipStream_t stream) {
+  if (K1 > kMaxPassBits || NB == 0) return hipErrorInvalidValue;
+  switch (gens.count) {
+#define MGBFS_G(k) \
+  case k:          \
+    return expand_route_g<k>(bank, F, gens, L, K1, NB, counts, scan, scan_ws, out, stream);
+    MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
+    MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
+#undef MGBFS_G
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* scan1, uint32_t NB1, uint32_t K1, uint32_t K2,
+                            uint32_t NC, uint32_t* counts, uint32_t* scan2, uint32_t* scan_ws, uint64_t* out,
+                            hipStream_t stream) {
+  if (K2 == 0 || K2 > kMaxPassBits || K1 + K2 > kMaxBucketBits || NC == 0) return hipErrorInvalidValue;
+  const uint32_t blocks = (1u << K1) * NC;
+  hipLaunchKernelGGL(partition_count_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, scan1, NB1, K1, K2, NC,
+                     counts);
+  hipError_t e = exclusive_scan_u32(counts, (static_cast<uint64_t>(1) << (K1 + K2)) * NC, scan2, scan_ws, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, scan1, NB1, K1, K2,
+                     NC, scan2, out);
+  return hipGetLastError();
+}
+
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, const uint64_t* cur,
+                              const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
+                              uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
+                              unsigned long long* unique_total, uint32_t tagbits, hipStream_t stream) {
+  if (K > Ko || Ko - K > kMaxFineBits || (cur && K > Kc) || (prev && K > Kp)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(unique_diff_kernel, dim3(1u << K), dim3(kUniqueThreads), 0, stream, cand, cscan, NC, K, cur,
+                     cur_off, Kc, prev, prev_off, Kp, Ko, fine_counts, error, unique_total, tagbits);
+  return hipGetLastError();
+}
+
+hipError_t sorted_compact(const uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, uint32_t Ko,
+                          const uint32_t* off, uint64_t* bank, hipStream_t stream) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, cscan, NC, K, Ko, off, bank);
+  return hipGetLastError();
+}
+
+}  // namespace mgbfs

@@ -1,0 +1,164 @@
+"""Parity of the packed HIP backends with the CPU oracle (through the C ABI).
+
+Layer tables must be bit-exact; for small graphs every layer's state set is
+compared (the reference compares full layer state sets on small runs,
+tests/segmented_single_gpu.py:40-43) and for large ones the layer table is
+compared with the oracle and with the reference's golden tables.
+"""
+import json
+import math
+import random
+from pathlib import Path
+
+import pytest
+
+from multigpubfs_amd import GraphDefinition, NativeEngine, lrx
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "lrx_layers.json").read_text())
+BACKENDS = ["permutation_bitmap", "sorted"]
+NAMES = {"permutation_bitmap": "PERMUTATION_BITMAP_LEHMER", "sorted": "SORTED_THREE_BANK_LDS_UNIQUE"}
+
+
+def run_all_layers(engine, keep_keys=False):
+    keys = [sorted(engine.layer_keys())] if keep_keys else None
+    while True:
+        status = engine.advance()
+        if status == 1:
+            break
+        assert status == 0, (status, engine.fatal)
+        if keep_keys:
+            keys.append(sorted(engine.layer_keys()))
+    return engine.layer_sizes(), keys
+
+
+def random_generators(rng, n, k, inverse_closed):
+    gens = []
+    for _ in range(k):
+        g = list(range(n))
+        rng.shuffle(g)
+        gens.append(g)
+    if inverse_closed:
+        for g in list(gens):
+            h = [0] * n
+            for i, v in enumerate(g):
+                h[v] = i
+            if h not in gens:
+                gens.append(h)
+    return gens
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_s8_lrx_layer_sets_bit_exact(backend):
+    ref = O.packed_bfs(8, keep_keys=True)
+    with NativeEngine(lrx(8), backend=backend) as e:
+        assert e.backend_name == NAMES[backend]
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+    assert sum(sizes) == 40320
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_s10_lrx_matches_reference_golden_table(backend):
+    with NativeEngine(lrx(10), backend=backend) as e:
+        sizes, _ = run_all_layers(e)
+    assert sizes == GOLDEN["layers"]["10"]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_s11_lrx_matches_oracle(backend):
+    ref = O.packed_bfs(11)["layers"]
+    with NativeEngine(lrx(11), backend=backend) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref
+    assert info["total_states"] == math.factorial(11)
+
+
+def test_s11_sorted_overflow_retry_path_is_exact():
+    # Coarse start buckets force sub-bucket overflows and finer retries.
+    ref = O.packed_bfs(11)["layers"]
+    with NativeEngine(lrx(11), backend="sorted", flags=1) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref
+    assert info["retries"] > 0
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
+def test_random_inverse_closed_sets_match_oracle(backend, seed):
+    rng = random.Random(seed)
+    n = rng.choice([7, 8, 9])
+    gens = random_generators(rng, n, rng.randint(1, 3), True)
+    start = list(range(n))
+    rng.shuffle(start)
+    ref = O.packed_bfs(n, gens, start=start, keep_keys=True)
+    with NativeEngine(GraphDefinition.permutation(gens, start), backend=backend) as e:
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_bitmap_directed_sets_match_oracle(seed):
+    rng = random.Random(seed)
+    n = rng.choice([7, 8, 9])
+    gens = random_generators(rng, n, rng.randint(2, 5), False)
+    ref = O.packed_bfs(n, gens, keep_keys=True)
+    with NativeEngine(GraphDefinition.permutation(gens, list(range(n))), backend="permutation_bitmap") as e:
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+
+
+def test_sorted_repeated_labels_match_ordered_set_oracle():
+    # Multiset start (repeated labels): outside the Lehmer bitmap domain.
+    rng = random.Random(5)
+    for start in ([0, 0, 1, 1, 2, 2, 3, 3, 4], [0, 0, 0, 1, 1, 2, 3, 4, 5, 6]):
+        n = len(start)
+        gens = O.lrx_generators(n) + random_generators(rng, n, 1, True)
+        graph = {"action": {"kind": "permutation", "degree": n, "generators": gens}, "start": start}
+        ref = O.c_exact_layers(graph, math.factorial(n))
+        with NativeEngine(GraphDefinition.permutation(gens, start), backend="sorted") as e:
+            sizes, keys = run_all_layers(e, keep_keys=True)
+        assert sizes == [len(x) for x in ref]
+        assert keys == [sorted(O.pack(s) for s in layer) for layer in ref]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_samples_match_last_and_previous_layers(backend):
+    ref = O.packed_bfs(8, keep_keys=True)
+    with NativeEngine(lrx(8), backend=backend) as e:
+        run_all_layers(e)
+        cur = e.sample(0, 1000)
+        prev = e.sample(1, 1000)
+    assert sorted(O.pack(r) for r in cur) == sorted(ref["keys"][-1])
+    assert sorted(O.pack(r) for r in prev) == sorted(ref["keys"][-2])
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_reset_repeats_identical_tables(backend):
+    with NativeEngine(lrx(9), backend=backend) as e:
+        first, _ = run_all_layers(e)
+        e.reset()
+        second, _ = run_all_layers(e)
+    assert first == second == O.packed_bfs(9)["layers"]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_capacity_limit_stops_with_resource(backend):
+    # A capacity below the widest layer must stop as RESOURCE, never silently.
+    with NativeEngine(lrx(8), backend=backend, capacity=100) as e:
+        statuses = []
+        while True:
+            s = e.advance()
+            statuses.append(s)
+            if s != 0:
+                break
+        assert statuses[-1] == 2 and e.fatal & 4
+        ref = O.packed_bfs(8)["layers"]
+        sizes = e.layer_sizes()
+        assert sizes == ref[:len(sizes)]
