@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (static_cast<uint32_t>(g) + 1 == skip) continue;
-      const uint64_t c = apply_packed(key, gens.src[g], n);
+      const uint64_t c = apply_groups(key, gens, g);
       bool bad;
       const uint64_t r = lehmer_rank_packed(c, n, &bad);
       if (bad || r >= bitmap_bits) {
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
         const uint32_t b = __ffs(m) - 1;
         m &= m - 1;
         const uint32_t it = b / G, g = b - it * G;
-        uint64_t c = apply_packed(parents[it][threadIdx.x], gens.src[g], n);
+        uint64_t c = apply_groups(parents[it][threadIdx.x], gens, g);
         if (gens.tag_shift < 64) c |= static_cast<uint64_t>(gens.child_tag[g]) << gens.tag_shift;
         next[pos++] = c;
       }

@@ -31,10 +31,34 @@ struct PackedGenerators {
   uint8_t src[kMaxGenerators][kMaxDegree];  // child[i] = parent[src[g][i]]
   uint8_t child_tag[kMaxGenerators];        // tag written into a child made by g
   uint32_t tag_shift;                        // bitmap backend: tag bit position (64 = off)
+  // The same permutation as shift groups: child nibble i takes parent nibble
+  // i + delta for every destination nibble in the group's mask, so a
+  // generator costs one shift and one mask per distinct displacement
+  // (2 for a rotation, 3 for a transposition of neighbours).
+  uint8_t groups[kMaxGenerators];
+  int8_t delta[kMaxGenerators][kMaxDegree];
+  uint64_t gmask[kMaxGenerators][kMaxDegree];
 };
 
-// Bijective mixer on B-bit keys (B = 4n): xorshift by B/2+1 (self-inverse
-// within B bits) and odd multipliers mod 2^B.  The sorted backend stores
+__host__ inline void fill_shift_groups(PackedGenerators& pg) {
+  for (uint32_t g = 0; g < pg.count; ++g) {
+    pg.groups[g] = 0;
+    for (uint32_t i = 0; i < pg.degree; ++i) {
+      const int d = static_cast<int>(pg.src[g][i]) - static_cast<int>(i);
+      uint32_t j = 0;
+      while (j < pg.groups[g] && pg.delta[g][j] != d) ++j;
+      if (j == pg.groups[g]) {
+        pg.delta[g][j] = static_cast<int8_t>(d);
+        pg.gmask[g][j] = 0;
+        ++pg.groups[g];
+      }
+      pg.gmask[g][j] |= 15ull << (4 * i);
+    }
+  }
+}
+
+// Bijective mixer on B-bit keys (B = 4n): an odd multiplier mod 2^B and an
+// xorshift by B/2+1 (self-inverse within B bits).  The sorted backend stores
 // (mix(key) << (64 - B)) | tag, so the top bits are uniform bucket ids.
 struct KeyMix {
   uint32_t bits, shift;
@@ -61,6 +85,18 @@ __device__ __forceinline__ uint64_t apply_packed(uint64_t key, const uint8_t* sr
     }
   }
   return (static_cast<uint64_t>(chi) << 32) | clo;
+}
+
+// Same result as apply_packed, from the shift groups of generator g.
+__device__ __forceinline__ uint64_t apply_groups(uint64_t key, const PackedGenerators& pg, int g) {
+  uint64_t child = 0;
+  const uint32_t groups = pg.groups[g];
+  for (uint32_t j = 0; j < groups; ++j) {
+    const int d = pg.delta[g][j];
+    const uint64_t moved = d >= 0 ? (key >> (4 * d)) : (key << (-4 * d));
+    child |= moved & pg.gmask[g][j];
+  }
+  return child;
 }
 
 __host__ inline uint64_t apply_packed_host(uint64_t key, const uint8_t* src, uint32_t n) {
@@ -123,21 +159,18 @@ __host__ __device__ __forceinline__ uint64_t unmix64(uint64_t x) {
   return x;
 }
 
+// Multiplicative (Fibonacci-style) hashing mod 2^B puts every key bit into
+// the top bits (the bucket ids); the final xorshift folds the top half into
+// the low bits used as LDS slot hashes.  Both steps are bijections on B bits.
 __host__ __device__ __forceinline__ uint64_t keymix(uint64_t x, const KeyMix& k) {
-  x ^= x >> k.shift;
   x = (x * k.m1) & k.mask;
-  x ^= x >> k.shift;
-  x = (x * k.m2) & k.mask;
   x ^= x >> k.shift;
   return x;
 }
 
 __host__ __device__ __forceinline__ uint64_t keyunmix(uint64_t x, const KeyMix& k) {
   x ^= x >> k.shift;
-  x = (x * k.i2) & k.mask;
-  x ^= x >> k.shift;
   x = (x * k.i1) & k.mask;
-  x ^= x >> k.shift;
   return x;
 }
 
@@ -146,10 +179,10 @@ __host__ inline KeyMix make_keymix(uint32_t bits) {
   k.bits = bits;
   k.shift = bits / 2 + 1;
   k.mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
-  k.m1 = 0xff51afd7ed558ccdull & k.mask;
-  k.m2 = 0xc4ceb9fe1a85ec53ull & k.mask;
-  k.i1 = inverse_odd(0xff51afd7ed558ccdull) & k.mask;
-  k.i2 = inverse_odd(0xc4ceb9fe1a85ec53ull) & k.mask;
+  // 2^64 / golden ratio, rounded to odd.
+  k.m1 = 0x9e3779b97f4a7c15ull & k.mask;
+  k.i1 = inverse_odd(0x9e3779b97f4a7c15ull) & k.mask;
+  k.m2 = k.i2 = 0;
   return k;
 }
 

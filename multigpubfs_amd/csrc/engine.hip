@@ -45,6 +45,7 @@ PackedGenerators Graph::packed_generators() const {
   pg.tag_shift = 64;
   for (uint32_t g = 0; g < generators; ++g)
     for (uint32_t i = 0; i < elements; ++i) pg.src[g][i] = static_cast<uint8_t>(perms[g * elements + i]);
+  fill_shift_groups(pg);
   return pg;
 }
 
@@ -491,12 +492,14 @@ extern "C" int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, 
   std::memset(&pg, 0, sizeof(pg));
   pg.count = generators;
   pg.degree = degree;
+  pg.tag_shift = 64;
   for (uint32_t g = 0; g < generators; ++g)
     for (uint32_t i = 0; i < degree; ++i) {
       const uint32_t v = permutations[g * degree + i];
       if (v >= degree) return hipErrorInvalidValue;
       pg.src[g][i] = static_cast<uint8_t>(v);
     }
+  mgbfs::fill_shift_groups(pg);
   return mgbfs::bitmap_expand(frontier, count, pg, bitmap, bitmap_bits, next,
                               reinterpret_cast<unsigned long long*>(next_count), next_capacity, error,
                               static_cast<hipStream_t>(stream));

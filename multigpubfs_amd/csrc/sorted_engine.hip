@@ -22,7 +22,7 @@ uint32_t ceil_log2(uint64_t x) {
   return b;
 }
 
-constexpr uint64_t kTargetBucket = 2048;  // average candidates per sub-bucket
+constexpr uint64_t kTargetBucket = 3072;  // average candidates per sub-bucket (cap 4096)
 constexpr uint32_t kPhaseCount = 4;       // expand, partition, unique, compact
 
 }  // namespace
@@ -113,7 +113,9 @@ class SortedEngine final : public Engine {
       return MGBFS_RESOURCE;
     }
     const int nxt = next_bank();
-    uint32_t K = Cmax <= kTargetBucket ? 0 : ceil_log2((Cmax + kTargetBucket - 1) / kTargetBucket);
+    // With parent-edge tags every tagged parent skips exactly one generator.
+    const uint64_t Cest = (layout_.prune && layers.size() > 1) ? F * (G - 1 ? G - 1 : 1) : Cmax;
+    uint32_t K = Cest <= kTargetBucket ? 0 : ceil_log2((Cest + kTargetBucket - 1) / kTargetBucket);
     // Test hook: start three bits coarse so large layers exercise the
     // bucket-overflow retry path.
     if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && K >= 3) K -= 3;
@@ -126,7 +128,7 @@ class SortedEngine final : public Engine {
       const uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 2048));
       const uint64_t n1 = (1ull << K1) * NB1;
       uint32_t NC = 1;
-      if (K2) NC = static_cast<uint32_t>(std::clamp<uint64_t>((Cmax >> K1) / 262144 + 1, 1, 16));
+      if (K2) NC = static_cast<uint32_t>(std::clamp<uint64_t>((Cest >> K1) / 262144 + 1, 1, 16));
       const uint64_t n2 = K2 ? (1ull << K) * NC : 0;
       ensure_u32(counts1_, n1);
       ensure_u32(scan1_, n1 + 1);

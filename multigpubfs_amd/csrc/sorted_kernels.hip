@@ -25,10 +25,12 @@
 
 namespace mgbfs {
 
-constexpr int kRouteThreads = 256;
+constexpr int kRouteThreads = 512;
 constexpr int kTile = 4096;
 constexpr int kTilePerThread = kTile / kRouteThreads;
-constexpr int kUniqueThreads = 256;
+constexpr int kUniqueThreads = 512;
+constexpr int kUniquePerThread = kUniqueCap / kUniqueThreads;  // candidates held in registers
+constexpr int kPrefetch = 4;  // independent loads issued before dependent LDS work
 
 __device__ __forceinline__ uint32_t top_bits(uint64_t h, uint32_t bits) {
   return bits ? static_cast<uint32_t>(h >> (64u - bits)) : 0u;
@@ -81,7 +83,7 @@ __device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t c
 // Stored value of a child made by generator g from `key`.
 __device__ __forceinline__ uint64_t child_value(uint64_t key, const PackedGenerators& gens, int g,
                                                 const SortedLayout& L) {
-  const uint64_t child = apply_packed(key, gens.src[g], gens.degree);
+  const uint64_t child = apply_groups(key, gens, g);
   return (keymix(child, L.km) << L.tagbits) | gens.child_tag[g];
 }
 
@@ -100,13 +102,23 @@ __global__ __launch_bounds__(kRouteThreads) void expand_count_kernel(const uint6
   for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const uint64_t p0 = F * blockIdx.x / NB, p1 = F * (blockIdx.x + 1) / NB;
-  for (uint64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const uint64_t v = bank[p];
-    const uint64_t key = keyunmix(v >> L.tagbits, L.km);
-    const uint32_t skip = parent_skip(v, L);
+  for (uint64_t base = p0 + threadIdx.x; base < p1; base += kPrefetch * kRouteThreads) {
+    uint64_t pv[kPrefetch];
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (static_cast<uint32_t>(g) + 1 != skip) atomicAdd(&hist[top_bits(child_value(key, gens, g, L), K1)], 1u);
+    for (int k = 0; k < kPrefetch; ++k) {
+      const uint64_t p = base + static_cast<uint64_t>(k) * kRouteThreads;
+      pv[k] = p < p1 ? bank[p] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      if (base + static_cast<uint64_t>(k) * kRouteThreads >= p1) break;
+      const uint64_t v = pv[k];
+      const uint64_t key = keyunmix(v >> L.tagbits, L.km);
+      const uint32_t skip = parent_skip(v, L);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (static_cast<uint32_t>(g) + 1 != skip) atomicAdd(&hist[top_bits(child_value(key, gens, g, L), K1)], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) counts[static_cast<uint64_t>(i) * NB + blockIdx.x] = hist[i];
@@ -176,8 +188,18 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     if (threadIdx.x == 0) t.fill = 0;
     __syncthreads();
     const uint32_t np = static_cast<uint32_t>(p1 - t0 < kParents ? p1 - t0 : kParents);
-    for (uint32_t q = threadIdx.x; q < np; q += kRouteThreads) {
-      const uint64_t v = bank[t0 + q];
+    constexpr int kPer = (kParents + kRouteThreads - 1) / kRouteThreads;
+    uint64_t pv[kPer];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const uint32_t q = m * kRouteThreads + threadIdx.x;
+      pv[m] = q < np ? bank[t0 + q] : 0;
+    }
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const uint32_t q = m * kRouteThreads + threadIdx.x;
+      if (q >= np) break;
+      const uint64_t v = pv[m];
       const uint64_t key = keyunmix(v >> L.tagbits, L.km);
       const uint32_t skip = parent_skip(v, L);
       uint32_t at = atomicAdd(&t.fill, skip ? G - 1u : static_cast<uint32_t>(G));
@@ -203,7 +225,17 @@ __global__ __launch_bounds__(kRouteThreads) void partition_count_kernel(const ui
   const uint32_t j = blockIdx.x / NC, c = blockIdx.x % NC;
   const uint64_t gs = scan1[static_cast<uint64_t>(j) * NB1], ge = scan1[static_cast<uint64_t>(j + 1) * NB1];
   const uint64_t r0 = gs + (ge - gs) * c / NC, r1 = gs + (ge - gs) * (c + 1) / NC;
-  for (uint64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) atomicAdd(&hist[mid_bits(in[i], K1, K2)], 1u);
+  for (uint64_t base = r0 + threadIdx.x; base < r1; base += kPrefetch * kRouteThreads) {
+    uint64_t v[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k) * kRouteThreads;
+      v[k] = i < r1 ? in[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k)
+      if (base + static_cast<uint64_t>(k) * kRouteThreads < r1) atomicAdd(&hist[mid_bits(v[k], K1, K2)], 1u);
+  }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < buckets; b += blockDim.x)
     counts[((static_cast<uint64_t>(j) << K2) + b) * NC + c] = hist[b];
@@ -252,31 +284,44 @@ struct UniqueLds {
 
 constexpr uint64_t kEmpty = ~0ull;
 
-__device__ __forceinline__ int lds_find(const UniqueLds& u, uint64_t h, uint32_t tagbits) {
-  uint32_t slot = static_cast<uint32_t>(h >> tagbits) & (kUniqueSlots - 1);
-  for (uint32_t probe = 0; probe < kUniqueSlots; ++probe) {
+__device__ __forceinline__ int lds_find(const UniqueLds& u, uint64_t h, uint32_t tagbits, uint32_t mask) {
+  uint32_t slot = static_cast<uint32_t>(h >> tagbits) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
     const uint64_t v = u.table[slot];
     if (v == kEmpty) return -1;
     if (same_state(v, h, tagbits)) return static_cast<int>(slot);
-    slot = (slot + 1) & (kUniqueSlots - 1);
+    slot = (slot + 1) & mask;
   }
   return -1;
 }
 
+// Marks table entries present in bank[lo, hi); loads are batched per thread.
 __device__ __forceinline__ void probe_bank(UniqueLds& u, const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
-                                           uint32_t tagbits) {
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += kUniqueThreads) {
-    const uint64_t h = bank[i];
-    if (h == kEmpty) {
-      atomicOr(&u.flags, 2u);
-      continue;
+                                           uint32_t tagbits, uint32_t mask) {
+  for (uint64_t base = lo + threadIdx.x; base < hi; base += kPrefetch * kUniqueThreads) {
+    uint64_t v[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k) * kUniqueThreads;
+      v[k] = i < hi ? bank[i] : 0;
     }
-    const int s = lds_find(u, h, tagbits);
-    if (s >= 0) atomicOr(&u.found[s >> 5], 1u << (s & 31));
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      if (base + static_cast<uint64_t>(k) * kUniqueThreads >= hi) break;
+      const uint64_t h = v[k];
+      if (h == kEmpty) {
+        atomicOr(&u.flags, 2u);
+        continue;
+      }
+      const int s = lds_find(u, h, tagbits, mask);
+      if (s >= 0) atomicOr(&u.found[s >> 5], 1u << (s & 31));
+    }
   }
 }
 
-// grid = 2^K blocks; block f handles sub-bucket f.
+// grid = 2^K blocks; block f handles sub-bucket f.  Each thread keeps its
+// (at most kUniquePerThread) candidates and their slots in registers, so
+// only the candidates it inserted first are revisited as survivors.
 __global__ __launch_bounds__(kUniqueThreads) void unique_diff_kernel(
     uint64_t* __restrict__ cand, const uint32_t* __restrict__ cscan, uint32_t NC, uint32_t K,
     const uint64_t* __restrict__ cur, const uint32_t* __restrict__ cur_off, uint32_t Kc,
@@ -294,53 +339,77 @@ __global__ __launch_bounds__(kUniqueThreads) void unique_diff_kernel(
       fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = 0;
     return;
   }
-  for (uint32_t i = threadIdx.x; i < kUniqueSlots; i += kUniqueThreads) u.table[i] = kEmpty;
-  for (uint32_t i = threadIdx.x; i < kUniqueSlots / 32; i += kUniqueThreads) u.found[i] = 0;
+  // Table sized to the bucket: slots = next_pow2(2c) (load factor <= 0.5).
+  uint32_t slots = 64;
+  while (slots < 2 * c) slots <<= 1;
+  const uint32_t mask = slots - 1;
+  uint64_t v[kUniquePerThread];
+#pragma unroll
+  for (int k = 0; k < kUniquePerThread; ++k) {
+    const uint64_t i = s + static_cast<uint64_t>(k) * kUniqueThreads + threadIdx.x;
+    v[k] = i < e ? cand[i] : 0;
+  }
+  for (uint32_t i = threadIdx.x; i < slots; i += kUniqueThreads) u.table[i] = kEmpty;
+  for (uint32_t i = threadIdx.x; i < slots / 32 + 1; i += kUniqueThreads) u.found[i] = 0;
   for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
   if (threadIdx.x == 0) {
     u.flags = 0;
     u.unique = 0;
   }
   __syncthreads();
-  // In-layer dedup: insert every candidate (full 64-bit equality).
-  uint32_t mine = 0;
-  for (uint64_t i = s + threadIdx.x; i < e; i += kUniqueThreads) {
-    const uint64_t h = cand[i];
+  // In-layer dedup: insert every candidate (state equality ignores tags).
+  uint32_t slot_of[kUniquePerThread];
+  uint32_t owned = 0;  // bit k: this thread inserted v[k] first
+  uint32_t special = 0;
+#pragma unroll
+  for (int k = 0; k < kUniquePerThread; ++k) {
+    slot_of[k] = 0;
+    if (s + static_cast<uint64_t>(k) * kUniqueThreads + threadIdx.x >= e) continue;
+    const uint64_t h = v[k];
     if (h == kEmpty) {
-      if (!(atomicOr(&u.flags, 1u) & 1u)) ++mine;
+      if (!(atomicOr(&u.flags, 1u) & 1u)) special = 1;
       continue;
     }
-    uint32_t slot = static_cast<uint32_t>(h >> tagbits) & (kUniqueSlots - 1);
+    uint32_t slot = static_cast<uint32_t>(h >> tagbits) & mask;
     while (true) {
       const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
       if (old == kEmpty) {
-        ++mine;
+        owned |= 1u << k;
+        slot_of[k] = slot;
         break;
       }
       if (same_state(old, h, tagbits)) break;
-      slot = (slot + 1) & (kUniqueSlots - 1);
+      slot = (slot + 1) & mask;
     }
   }
+  const uint32_t mine = __popc(owned) + special;
   if (mine) atomicAdd(&u.unique, mine);
   __syncthreads();
   // Difference against the two retained layers (same sub-bucket only).
   if (cur) {
     const uint32_t sh = Kc - K;
-    probe_bank(u, cur, cur_off[static_cast<uint64_t>(f) << sh], cur_off[static_cast<uint64_t>(f + 1) << sh], tagbits);
+    probe_bank(u, cur, cur_off[static_cast<uint64_t>(f) << sh], cur_off[static_cast<uint64_t>(f + 1) << sh], tagbits,
+               mask);
   }
   if (prev) {
     const uint32_t sh = Kp - K;
     probe_bank(u, prev, prev_off[static_cast<uint64_t>(f) << sh], prev_off[static_cast<uint64_t>(f + 1) << sh],
-               tagbits);
+               tagbits, mask);
   }
   __syncthreads();
-  const bool max_survives = (u.flags & 3u) == 1u;
-  // Survivors grouped by Ko fine bits.
-  for (uint32_t slot = threadIdx.x; slot < kUniqueSlots; slot += kUniqueThreads) {
-    const uint64_t h = u.table[slot];
-    if (h != kEmpty && !((u.found[slot >> 5] >> (slot & 31)) & 1u)) atomicAdd(&u.fhist[mid_bits(h, K, fine_bits)], 1u);
+  const bool max_survives = special && !(u.flags & 2u);
+  uint32_t survive = 0;
+#pragma unroll
+  for (int k = 0; k < kUniquePerThread; ++k) {
+    if ((owned >> k) & 1u) {
+      const uint32_t sl = slot_of[k];
+      if (!((u.found[sl >> 5] >> (sl & 31)) & 1u)) {
+        survive |= 1u << k;
+        atomicAdd(&u.fhist[mid_bits(v[k], K, fine_bits)], 1u);
+      }
+    }
   }
-  if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
+  if (max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
   __syncthreads();
   lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
   for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads)
@@ -348,18 +417,12 @@ __global__ __launch_bounds__(kUniqueThreads) void unique_diff_kernel(
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = u.fstart[i];
   __syncthreads();
-  // All candidates are in LDS: write survivors back in place at cand[s..).
-  for (uint32_t slot = threadIdx.x; slot < kUniqueSlots; slot += kUniqueThreads) {
-    const uint64_t h = u.table[slot];
-    if (h != kEmpty && !((u.found[slot >> 5] >> (slot & 31)) & 1u)) {
-      const uint32_t pos = atomicAdd(&u.fhist[mid_bits(h, K, fine_bits)], 1u);
-      cand[s + pos] = h;
-    }
-  }
-  if (threadIdx.x == 0) {
-    if (max_survives) cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
-    atomicAdd(unique_total, static_cast<unsigned long long>(u.unique));
-  }
+  // Every candidate is in registers: write survivors back in place at cand[s..).
+#pragma unroll
+  for (int k = 0; k < kUniquePerThread; ++k)
+    if ((survive >> k) & 1u) cand[s + atomicAdd(&u.fhist[mid_bits(v[k], K, fine_bits)], 1u)] = v[k];
+  if (max_survives) cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
+  if (threadIdx.x == 0) atomicAdd(unique_total, static_cast<unsigned long long>(u.unique));
 }
 
 // grid = 2^K blocks: survivors of sub-bucket f -> bank rows.
