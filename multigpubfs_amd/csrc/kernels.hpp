@@ -32,15 +32,16 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out, uin
 hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
                        hipStream_t stream);
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan, uint32_t* scan_ws,
-                               uint64_t* out, hipStream_t stream);
+                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan,
+                               uint32_t* scan_ws, uint64_t* out, hipStream_t stream);
 hipError_t sorted_partition(const uint64_t* in, const uint32_t* scan1, uint32_t NB1, uint32_t K1, uint32_t K2,
                             uint32_t NC, uint32_t* counts, uint32_t* scan2, uint32_t* scan_ws, uint64_t* out,
                             hipStream_t stream);
 hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, const uint64_t* cur,
                               const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
-                              unsigned long long* unique_total, uint32_t tagbits, hipStream_t stream);
+                              unsigned long long* unique_total, uint32_t tagbits, uint32_t grid_limit,
+                              hipStream_t stream);
 hipError_t sorted_compact(const uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, uint32_t Ko,
                           const uint32_t* off, uint64_t* bank, hipStream_t stream);
 

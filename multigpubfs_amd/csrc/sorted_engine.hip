@@ -50,12 +50,18 @@ class SortedEngine final : public Engine {
     layout_.tagbits = 64 - 4 * n_;
     layout_.tagmask = layout_.tagbits ? ((1ull << layout_.tagbits) - 1) : 0;
     const std::vector<int> inv = inverse_index(gens_);
+    for (int x : inv)
+      if (x < 0) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
     // Tags 1..G must leave the all-ones tag value unused (kEmpty sentinel).
     layout_.prune = layout_.tagbits >= 5 && gens_.count + 1 < (1ull << layout_.tagbits) - 1;
     for (uint32_t a = 0; a < gens_.count; ++a)
       gens_.child_tag[a] = (layout_.prune && inv[a] >= 0) ? static_cast<uint8_t>(inv[a] + 1) : 0;
     margin_ = std::min<uint32_t>(kMaxFineBits, 2 * (ceil_log2(gens_.count) + 1));
     start_key_ = g.start_key();
+    // One workgroup per sub-bucket measured faster than a persistent grid
+    // with next-bucket prefetch (the kernel is bound by LDS atomics, not by
+    // load latency), so the grid is not capped.
+    unique_grid_ = 1u << kMaxBucketBits;
     control_.allocate(64);
     fine_counts_.allocate(((1ull << kMaxBucketBits) + 1) * 4);
     for (int b = 0; b < 3; ++b) off_[b].allocate(((1ull << kMaxBucketBits) + 1) * 4);
@@ -140,7 +146,9 @@ class SortedEngine final : public Engine {
       const uint32_t Ko = std::min<uint32_t>(kMaxBucketBits, K + margin_);
       MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 2, stream_));
       MGBFS_HIP(hipEventRecord(ev_[0], stream_));
-      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, K1, NB1, counts1_.as<uint32_t>(),
+      const bool all_tagged = layout_.prune && layers.size() > 1;
+      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, all_tagged, K1, NB1,
+                                    counts1_.as<uint32_t>(),
                                     scan1_.as<uint32_t>(), scan_ws_.as<uint32_t>(), cand1_.as<uint64_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[1], stream_));
       uint64_t* seg = cand1_.as<uint64_t>();
@@ -160,7 +168,7 @@ class SortedEngine final : public Engine {
           seg, segscan, segNC, K, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), ko_[cur_],
           has_prev ? bank_[prev_].as<uint64_t>() : nullptr, has_prev ? off_[prev_].as<uint32_t>() : nullptr,
           has_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(),
-          reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()), layout_.tagbits, stream_));
+          reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()), layout_.tagbits, unique_grid_, stream_));
       MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), 1ull << Ko, off_[nxt].as<uint32_t>(),
                                    scan_ws_.as<uint32_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[3], stream_));
@@ -322,7 +330,7 @@ class SortedEngine final : public Engine {
 
   PackedGenerators gens_;
   SortedLayout layout_{};
-  uint32_t n_, margin_ = 0, flags_ = 0;
+  uint32_t n_, margin_ = 0, flags_ = 0, unique_grid_ = 2048;
   uint64_t start_key_ = 0, budget_ = 0, cap_limit_ = 0, cand_cap_ = 0;
   DeviceBuffer bank_[3], off_[3], cand1_, cand2_, counts1_, scan1_, counts2_, scan2_, scan_ws_, fine_counts_, control_;
   uint64_t bank_cap_[3] = {0, 0, 0}, count_[3] = {0, 0, 0};
