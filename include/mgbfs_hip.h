@@ -51,6 +51,8 @@ extern "C" {
 /* Option flags. */
 #define MGBFS_FLAG_TEST_COARSE_BUCKETS 1u /* sorted backend: start 3 bucket bits coarse
                                              so large layers take the overflow-retry path */
+#define MGBFS_FLAG_TEST_COLLIDE_HASH 2u   /* generic backend: hash bits 0, every state
+                                             collides and full-state equality decides */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are
@@ -67,6 +69,7 @@ typedef struct mgbfs_graph_desc {
   const int64_t* matrices;      /* kind 1: [generators][rows*rows]          */
   const uint32_t* moduli;       /* kind 1: [generators], 0 = no reduction   */
   const int64_t* start;         /* [elements]                               */
+  uint64_t expected_states;     /* expected_max_unique_states (0 = none)    */
 } mgbfs_graph_desc;
 
 typedef struct mgbfs_engine_options {
@@ -142,6 +145,52 @@ int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint3
                                const uint64_t* frontier, uint64_t count, uint32_t* bitmap, uint64_t bitmap_bits,
                                uint64_t* next, uint64_t* next_count, uint64_t next_capacity, uint32_t* error,
                                void* stream);
+
+/* --------------------------------------------- generic SoA kernel ABI */
+/* Same signatures and argument meaning as the reference's libmgbfs_cuda
+ * (cuda/generic_visited.h:3-11 and :22-44, cuda/generic_graph.h:3-6):
+ * states are element-major SoA planes (int64 or uint8) with a row stride;
+ * `slots` is a power-of-two open-addressing table initialised to all ones;
+ * `future` receives arena row indices; counts and `error` are device words.
+ * kind 0 = permutation (rows = degree, cols = 1), kind 1 = int64 matrix
+ * action rows x cols with per-generator modulus (0 = wrap only).  The uint8
+ * entry points accept permutation actions only.  hash_bits 0..64 (0 forces
+ * every state to collide; equality always decides). */
+int mgbfs_generic_seed_i64_wide(uint32_t elements, const int64_t* states, uint32_t state_stride, uint32_t state_count,
+                                uint64_t* slots, uint64_t slot_capacity, uint64_t seed, uint32_t hash_bits,
+                                uint32_t* error, void* stream);
+int mgbfs_generic_seed_u8_wide(uint32_t elements, const uint8_t* states, uint32_t stride, uint32_t count,
+                               uint64_t* slots, uint64_t capacity, uint64_t seed, uint32_t bits, uint32_t* error,
+                               void* stream);
+int mgbfs_generic_expand_i64_wide(uint32_t kind, uint32_t elements, uint32_t rows, uint32_t cols, uint32_t generators,
+                                  const int64_t* parents, uint32_t parent_count, uint32_t parent_stride,
+                                  const uint32_t* permutation_tables, const int64_t* matrix_tables,
+                                  const uint32_t* moduli, uint64_t* slots, uint64_t slot_capacity, int64_t* visited,
+                                  uint32_t visited_capacity, uint32_t* visited_count, uint32_t* future,
+                                  uint32_t future_capacity, uint32_t* future_count, uint64_t seed, uint32_t hash_bits,
+                                  uint32_t* error, void* stream);
+int mgbfs_generic_expand_u8_wide(uint32_t kind, uint32_t elements, uint32_t n, uint32_t m, uint32_t generators,
+                                 const uint8_t* parents, uint32_t count, uint32_t stride, const uint32_t* permutations,
+                                 const int64_t* matrices, const uint32_t* moduli, uint64_t* slots,
+                                 uint64_t slot_capacity, uint8_t* visited, uint32_t visited_capacity,
+                                 uint32_t* visited_count, uint32_t* future, uint32_t future_capacity,
+                                 uint32_t* future_count, uint64_t seed, uint32_t bits, uint32_t* error, void* stream);
+int mgbfs_generic_gather_i64(uint32_t elements, const int64_t* source, uint32_t source_stride,
+                             const uint32_t* indices, uint32_t count, int64_t* output, uint32_t output_stride,
+                             void* stream);
+int mgbfs_generic_gather_u8(uint32_t elements, const uint8_t* source, uint32_t stride, const uint32_t* indices,
+                            uint32_t count, uint8_t* output, uint32_t output_stride, void* stream);
+/* Materialises selected children (index = parent * generators + generator). */
+int mgbfs_generic_generate_i64(uint32_t kind, uint32_t elements, uint32_t rows, uint32_t cols, uint32_t generators,
+                               const int64_t* parents, uint32_t parent_count, uint32_t parent_stride,
+                               const uint32_t* permutation_tables, const int64_t* matrix_tables,
+                               const uint32_t* moduli, const uint64_t* selected_children, uint32_t output_count,
+                               int64_t* output, uint32_t output_stride, uint32_t* device_error, void* stream);
+int mgbfs_generic_generate_u8(uint32_t kind, uint32_t elements, uint32_t rows, uint32_t cols, uint32_t generators,
+                              const uint8_t* parents, uint32_t parent_count, uint32_t parent_stride,
+                              const uint32_t* permutation_tables, const int64_t* matrix_tables,
+                              const uint32_t* moduli, const uint64_t* selected_children, uint32_t output_count,
+                              uint8_t* output, uint32_t output_stride, uint32_t* device_error, void* raw_stream);
 
 /* ---------------------------------------------------------- versions */
 uint32_t mgbfs_hip_abi_version(void);

@@ -25,7 +25,8 @@ class GraphDesc(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("elements", C.c_uint32), ("rows", C.c_uint32), ("cols", C.c_uint32),
                 ("generators", C.c_uint32), ("inverse_closed", C.c_uint32),
                 ("permutations", C.POINTER(C.c_uint32)), ("matrices", C.POINTER(C.c_int64)),
-                ("moduli", C.POINTER(C.c_uint32)), ("start", C.POINTER(C.c_int64))]
+                ("moduli", C.POINTER(C.c_uint32)), ("start", C.POINTER(C.c_int64)),
+                ("expected_states", C.c_uint64)]
 
 
 class EngineOptions(C.Structure):
@@ -109,6 +110,7 @@ class NativeEngine:
         desc.elements = self.elements
         desc.generators = len(a["generators"])
         desc.inverse_closed = int(gd.inverse_closed() if inverse_closed is None else inverse_closed)
+        desc.expected_states = data["expected_max_unique_states"] or 0
         start = _array(C.c_int64, data["start"])
         desc.start = start
         self._keep.append(start)

@@ -296,6 +296,7 @@ static Graph intake(const mgbfs_graph_desc* d) {
   g.cols = d->cols;
   g.generators = d->generators;
   g.inverse_closed = d->inverse_closed != 0;
+  g.expected_states = d->expected_states;
   g.start.assign(d->start, d->start + d->elements);
   if (d->kind == 0) {
     if (!d->permutations) throw std::runtime_error("GRAPH_PERMUTATION_GENERATOR");
@@ -322,12 +323,68 @@ static Graph intake(const mgbfs_graph_desc* d) {
   return g;
 }
 
+// Bound used for admission only (crates/mgbfs-runtime/src/generic_memory.rs:82-108):
+// permutations cannot move a label between position orbits, so the number of
+// arrangements of the start labels within each orbit bounds the state count;
+// a single nonzero modulus bounds matrices by m^elements (+1 for a
+// non-canonical start).  The caller's expected_max_unique_states caps both.
+uint64_t state_space_bound(const Graph& g) {
+  const uint64_t limit = 1ull << 63;
+  uint64_t bound = 0;
+  if (g.kind == 0) {
+    std::vector<uint32_t> parent(g.elements);
+    for (uint32_t i = 0; i < g.elements; ++i) parent[i] = i;
+    auto root = [&](uint32_t x) {
+      while (parent[x] != x) x = parent[x] = parent[parent[x]];
+      return x;
+    };
+    for (uint32_t k = 0; k < g.generators; ++k)
+      for (uint32_t i = 0; i < g.elements; ++i) parent[root(i)] = root(g.perms[k * g.elements + i]);
+    // Multinomial per orbit, computed as a product of binomials.
+    std::vector<std::vector<int64_t>> labels(g.elements);
+    for (uint32_t i = 0; i < g.elements; ++i) labels[root(i)].push_back(g.start[i]);
+    long double total = 1;
+    for (auto& l : labels) {
+      if (l.empty()) continue;
+      std::sort(l.begin(), l.end());
+      long double ways = 1;
+      uint64_t placed = 0;
+      for (size_t a = 0; a < l.size();) {
+        size_t b = a;
+        while (b < l.size() && l[b] == l[a]) ++b;
+        for (size_t t = 1; t <= b - a; ++t) ways = ways * static_cast<long double>(placed + t) / static_cast<long double>(t);
+        placed += b - a;
+        a = b;
+      }
+      total *= ways;
+      if (total > static_cast<long double>(limit)) break;
+    }
+    bound = total > static_cast<long double>(limit) ? limit : static_cast<uint64_t>(total + 0.5L);
+  } else {
+    const uint32_t m = g.moduli.empty() ? 0 : g.moduli[0];
+    bool same = m != 0;
+    for (uint32_t x : g.moduli) same &= x == m;
+    if (same) {
+      long double total = 1;
+      for (uint32_t e = 0; e < g.elements && total <= static_cast<long double>(limit); ++e) total *= m;
+      bound = total > static_cast<long double>(limit) ? limit : static_cast<uint64_t>(total);
+      for (int64_t v : g.start)
+        if (v < 0 || v >= static_cast<int64_t>(m)) {
+          bound = bound < limit ? bound + 1 : limit;
+          break;
+        }
+    }
+  }
+  if (g.expected_states) bound = bound ? std::min(bound, g.expected_states) : g.expected_states;
+  return bound;
+}
+
 static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
   uint32_t backend = o.backend;
   if (backend == MGBFS_BACKEND_AUTO) {
     if (g.packable() && g.inverse_closed) backend = MGBFS_BACKEND_SORTED;
     else if (g.packable() && g.distinct_perm()) backend = MGBFS_BACKEND_BITMAP;
-    else throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
+    else backend = MGBFS_BACKEND_GENERIC;
   }
   switch (backend) {
     case MGBFS_BACKEND_BITMAP:
@@ -336,6 +393,8 @@ static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
     case MGBFS_BACKEND_SORTED:
       if (!g.packable()) throw std::runtime_error("SORTED_UNSUPPORTED_GRAPH");
       return make_sorted_engine(g, o);
+    case MGBFS_BACKEND_GENERIC:
+      return make_generic_engine(g, o, state_space_bound(g));
     default:
       throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
   }

@@ -62,6 +62,7 @@ struct Graph {
   std::vector<int64_t> matrices;  // [G][rows*rows]
   std::vector<uint32_t> moduli;   // [G]
   std::vector<int64_t> start;     // [elements]
+  uint64_t expected_states = 0;   // expected_max_unique_states (0 = none)
 
   bool packable() const;      // permutation, n <= 16, all start values in 0..15
   bool distinct_perm() const; // start is a permutation of 0..n-1
@@ -102,6 +103,9 @@ class Engine {
 
 Engine* make_bitmap_engine(const Graph& g, const mgbfs_engine_options& o);
 Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o);
+Engine* make_generic_engine(const Graph& g, const mgbfs_engine_options& o, uint64_t state_bound);
+// Mathematical bound on reachable states (0 = unknown), capped at 2^63.
+uint64_t state_space_bound(const Graph& g);
 // Index of each generator's inverse in the set (-1 when absent).
 std::vector<int> inverse_index(const PackedGenerators& gens);
 
