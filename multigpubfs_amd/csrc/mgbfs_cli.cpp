@@ -1,0 +1,510 @@
+// mgbfs — command-line host of the MI355X exact BFS engine.
+//
+// Command grammar and output contract of the reference executable
+// (crates/mgbfs-cli/src/main.rs:20-199, crates/mgbfs-runtime/src/generic_run.rs:9-43,
+// generic_distributed_run.rs:13-73):
+//   mgbfs graph <graph.json> <out-dir> [--device N] [--capacity N] [--seconds N] [--backend NAME]
+//   mgbfs graph-info <graph.json> <devices> [shards] [capacity] [batch]
+//   mgbfs graph-count
+//   mgbfs graph-digest <graph.json>      (validation, digest, retention rule; no device)
+//   mgbfs key-info
+//   mgbfs version
+// Errors: nonzero exit and a last stderr line {"status":"ERROR","error":"CODE"}.
+// `graph` writes report.json (exact layer sizes) and states.json (<=1000 rows
+// of the last layer, the previous layer iff it has < 1000 rows); SIGTERM or
+// SIGINT ends the run as INCOMPLETE/CANCELLED with the completed prefix.
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <chrono>
+#include <csignal>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+#include <hip/hip_runtime.h>
+#include "../../include/mgbfs_hip.h"
+#include "host_util.hpp"
+
+using mgbfs::host::Json;
+
+namespace {
+
+volatile std::sig_atomic_t g_cancel = 0;
+extern "C" void on_signal(int) { g_cancel = 1; }
+
+struct CliError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+const int64_t kI64Max = INT64_MAX;
+
+int64_t wrap_mul_add(int64_t acc, int64_t a, int64_t b) {
+  return static_cast<int64_t>(static_cast<uint64_t>(acc) + static_cast<uint64_t>(a) * static_cast<uint64_t>(b));
+}
+
+int64_t pymod(int64_t x, int64_t m) {
+  int64_t r = x % m;
+  return r < 0 ? r + m : r;
+}
+
+// Validated schema-2 graph (multigpubfs/graph_definition.py:46-69).
+struct GraphDoc {
+  Json raw;
+  uint32_t kind = 0, elements = 0, rows = 0, cols = 0;
+  std::vector<std::vector<uint32_t>> perms;
+  std::vector<std::vector<int64_t>> mats;
+  std::vector<uint32_t> moduli;
+  std::vector<int64_t> start;
+  uint64_t expected = 0;
+
+  static std::vector<int64_t> ints(const Json& j) {
+    if (j.kind != Json::Arr) throw CliError("GRAPH_VECTOR");
+    std::vector<int64_t> v;
+    for (const Json& x : j.a) {
+      if (!x.is_int()) throw CliError("GRAPH_INT64_RANGE");
+      v.push_back(x.as_int());
+    }
+    return v;
+  }
+
+  explicit GraphDoc(const Json& j) : raw(j) {
+    static const char* keys[] = {"action", "expected_max_unique_states", "generator_names", "name", "schema", "start"};
+    if (j.kind != Json::Obj || j.o.size() != 6) throw CliError("GRAPH_SCHEMA");
+    for (const char* k : keys)
+      if (!j.has(k)) throw CliError("GRAPH_SCHEMA");
+    if (!j.at("schema").is_int() || j.at("schema").as_int() != 2) throw CliError("GRAPH_SCHEMA");
+    if (j.at("name").kind != Json::Str) throw CliError("GRAPH_NAME");
+    start = ints(j.at("start"));
+    const Json& a = j.at("action");
+    if (a.kind != Json::Obj || !a.has("generators") || a.at("generators").kind != Json::Arr ||
+        a.at("generators").a.empty())
+      throw CliError("GRAPH_GENERATORS");
+    const Json& names = j.at("generator_names");
+    if (names.kind != Json::Arr || names.a.size() != a.at("generators").a.size()) throw CliError("GRAPH_GENERATOR_NAMES");
+    for (const Json& n : names.a)
+      if (n.kind != Json::Str) throw CliError("GRAPH_GENERATOR_NAMES");
+    const Json& bound = j.at("expected_max_unique_states");
+    if (bound.kind != Json::Null) {
+      if (bound.kind == Json::UInt && bound.u >= 1) expected = bound.u;
+      else if (bound.kind == Json::Int && bound.i >= 1) expected = static_cast<uint64_t>(bound.i);
+      else throw CliError("GRAPH_ORBIT_BOUND");
+    }
+    const std::string kind_s = a.has("kind") && a.at("kind").kind == Json::Str ? a.at("kind").s : "";
+    if (kind_s == "permutation") {
+      if (a.o.size() != 3 || !a.has("degree") || !a.at("degree").is_int()) throw CliError("GRAPH_PERMUTATION_SHAPE");
+      const int64_t n = a.at("degree").as_int();
+      if (n < 1 || n > 0xffffffffll || static_cast<size_t>(n) != start.size()) throw CliError("GRAPH_PERMUTATION_SHAPE");
+      kind = 0;
+      elements = rows = static_cast<uint32_t>(n);
+      cols = 1;
+      for (const Json& gj : a.at("generators").a) {
+        std::vector<int64_t> g = ints(gj);
+        std::vector<int64_t> sorted_g = g;
+        std::sort(sorted_g.begin(), sorted_g.end());
+        for (int64_t i = 0; i < n; ++i)
+          if (static_cast<int64_t>(sorted_g.size()) != n || sorted_g[i] != i) throw CliError("GRAPH_PERMUTATION_GENERATOR");
+        perms.emplace_back(g.begin(), g.end());
+      }
+    } else if (kind_s == "matrix") {
+      if (a.o.size() != 4 || !a.has("rows") || !a.has("cols") || !a.at("rows").is_int() || !a.at("cols").is_int())
+        throw CliError("GRAPH_MATRIX_SHAPE");
+      const int64_t r = a.at("rows").as_int(), c = a.at("cols").as_int();
+      if (r < 1 || c < 1 || r * c > 0xffffffffll || static_cast<size_t>(r * c) != start.size())
+        throw CliError("GRAPH_MATRIX_SHAPE");
+      kind = 1;
+      rows = static_cast<uint32_t>(r);
+      cols = static_cast<uint32_t>(c);
+      elements = rows * cols;
+      for (const Json& gj : a.at("generators").a) {
+        if (gj.kind != Json::Obj || gj.o.size() != 2 || !gj.has("matrix") || !gj.has("modulo"))
+          throw CliError("GRAPH_MATRIX_GENERATOR");
+        std::vector<int64_t> m = ints(gj.at("matrix"));
+        if (m.size() != static_cast<size_t>(r * r)) throw CliError("GRAPH_MATRIX_GENERATOR");
+        if (!gj.at("modulo").is_int()) throw CliError("GRAPH_MATRIX_MODULUS");
+        const int64_t mod = gj.at("modulo").as_int();
+        if (mod != 0 && (mod < 2 || mod > (1ll << 31))) throw CliError("GRAPH_MATRIX_MODULUS");
+        mats.push_back(m);
+        moduli.push_back(static_cast<uint32_t>(mod));
+      }
+    } else {
+      throw CliError("GRAPH_ACTION_KIND");
+    }
+  }
+
+  // sha256 of canonical JSON {schema, action, start} (graph_definition.py:43-45).
+  std::string digest() const {
+    Json core = Json::object();
+    core["schema"] = raw.at("schema");
+    core["action"] = raw.at("action");
+    core["start"] = raw.at("start");
+    return mgbfs::host::sha256_hex(mgbfs::host::dumps(core));
+  }
+
+  std::vector<int64_t> compose(const std::vector<int64_t>& g, const std::vector<int64_t>& h, int64_t mod) const {
+    const uint32_t n = rows;
+    std::vector<int64_t> right(h);
+    if (mod)
+      for (auto& v : right) v = pymod(v, mod);
+    std::vector<int64_t> out(n * n);
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint32_t j = 0; j < n; ++j) {
+        int64_t acc = 0;
+        for (uint32_t k = 0; k < n; ++k) acc = wrap_mul_add(acc, g[i * n + k], right[k * n + j]);
+        out[i * n + j] = mod ? pymod(acc, mod) : acc;
+      }
+    return out;
+  }
+
+  // Retention rule of graph_definition.py:84-108.
+  bool inverse_closed() const {
+    if (kind == 0) {
+      for (const auto& g : perms) {
+        std::vector<uint32_t> inv(g.size());
+        for (size_t i = 0; i < g.size(); ++i) inv[g[i]] = static_cast<uint32_t>(i);
+        if (std::find(perms.begin(), perms.end(), inv) == perms.end()) return false;
+      }
+      return true;
+    }
+    for (uint32_t m : moduli)
+      if (m != moduli[0]) return false;
+    const int64_t mod = moduli[0];
+    const uint32_t n = rows;
+    if (mod && (mod & (mod - 1))) {
+      for (const auto& g : mats)
+        for (uint32_t i = 0; i < n; ++i) {
+          long double sum = 0;
+          for (uint32_t k = 0; k < n; ++k) sum += std::abs(static_cast<long double>(g[i * n + k])) * (mod - 1);
+          if (sum > static_cast<long double>(kI64Max)) return false;
+        }
+    }
+    std::vector<int64_t> eye(n * n, 0);
+    for (uint32_t i = 0; i < n; ++i) eye[i * n + i] = 1;
+    for (const auto& g : mats) {
+      bool found = false;
+      for (const auto& h : mats)
+        if (compose(g, h, mod) == eye && compose(h, g, mod) == eye) {
+          found = true;
+          break;
+        }
+      if (!found) return false;
+    }
+    if (mod)
+      for (int64_t v : start)
+        if (v < 0 || v >= mod) return false;
+    return true;
+  }
+
+  mgbfs_graph_desc desc(std::vector<uint32_t>& pflat, std::vector<int64_t>& mflat) const {
+    mgbfs_graph_desc d{};
+    d.kind = kind;
+    d.elements = elements;
+    d.rows = rows;
+    d.cols = cols;
+    d.generators = static_cast<uint32_t>(kind == 0 ? perms.size() : mats.size());
+    d.inverse_closed = inverse_closed() ? 1u : 0u;
+    for (const auto& g : perms) pflat.insert(pflat.end(), g.begin(), g.end());
+    for (const auto& m : mats) mflat.insert(mflat.end(), m.begin(), m.end());
+    d.permutations = pflat.empty() ? nullptr : pflat.data();
+    d.matrices = mflat.empty() ? nullptr : mflat.data();
+    d.moduli = moduli.empty() ? nullptr : moduli.data();
+    d.start = start.data();
+    d.expected_states = expected;
+    return d;
+  }
+};
+
+std::string read_file(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw CliError("CLI_GRAPH_READ");
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return ss.str();
+}
+
+void write_file(const std::string& path, const std::string& text) {
+  const std::string tmp = path + ".tmp";
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) throw CliError("CLI_OUTPUT_WRITE");
+    f << text;
+  }
+  if (std::rename(tmp.c_str(), path.c_str())) throw CliError("CLI_OUTPUT_WRITE");
+}
+
+uint64_t parse_u64(const std::string& s, const char* code) {
+  if (s.empty() || !std::all_of(s.begin(), s.end(), [](char c) { return std::isdigit(static_cast<unsigned char>(c)); }))
+    throw CliError(code);
+  errno = 0;
+  const unsigned long long v = std::strtoull(s.c_str(), nullptr, 10);
+  if (errno) throw CliError(code);
+  return v;
+}
+
+uint32_t backend_code(const std::string& name) {
+  if (name == "auto") return MGBFS_BACKEND_AUTO;
+  if (name == "permutation_bitmap" || name == "bitmap") return MGBFS_BACKEND_BITMAP;
+  if (name == "sorted") return MGBFS_BACKEND_SORTED;
+  if (name == "generic" || name == "legacy") return MGBFS_BACKEND_GENERIC;
+  throw CliError("CLI_GRAPH_BACKEND");
+}
+
+Json rows_json(const std::vector<int64_t>& flat, uint64_t count, uint32_t width) {
+  Json arr = Json::array();
+  for (uint64_t r = 0; r < count; ++r) {
+    Json row = Json::array();
+    for (uint32_t e = 0; e < width; ++e) row.push(Json::integer(flat[r * width + e]));
+    arr.push(row);
+  }
+  return arr;
+}
+
+int cmd_graph(const std::vector<std::string>& args) {
+  if (args.size() < 2)
+    throw CliError("CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N]");
+  int32_t device = 0;
+  uint64_t capacity = 0, seconds = 0;
+  std::string backend = "auto";
+  for (size_t i = 2; i < args.size(); i += 2) {
+    if (i + 1 >= args.size()) throw CliError("CLI_GRAPH_OPTION_VALUE");
+    const std::string& k = args[i];
+    if (k == "--device") device = static_cast<int32_t>(parse_u64(args[i + 1], "CLI_GRAPH_DEVICE"));
+    else if (k == "--capacity") capacity = parse_u64(args[i + 1], "CLI_GRAPH_CAPACITY");
+    else if (k == "--seconds") seconds = parse_u64(args[i + 1], "CLI_GRAPH_SECONDS");
+    else if (k == "--backend") backend = args[i + 1];
+    else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
+  }
+  const char* world = std::getenv("WORLD_SIZE");
+  if (world && std::string(world) != "1") throw CliError("GENERIC_DISTRIBUTED_INTEGRATION_NOT_READY");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  const std::string out = args[1];
+  struct stat st;
+  if (stat(out.c_str(), &st) == 0) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
+  if (mkdir(out.c_str(), 0755)) throw CliError("CLI_GRAPH_OUTPUT_CREATE");
+
+  std::vector<uint32_t> pflat;
+  std::vector<int64_t> mflat;
+  const mgbfs_graph_desc desc = graph.desc(pflat, mflat);
+  mgbfs_engine_options opts{};
+  opts.device = device;
+  opts.backend = backend_code(backend);
+  opts.capacity = capacity;
+  void* engine = nullptr;
+  char err[256] = {0};
+  const auto setup0 = std::chrono::steady_clock::now();
+  if (mgbfs_engine_create(&desc, &opts, &engine, err, sizeof err)) throw CliError(err[0] ? err : "ENGINE_CREATE");
+  const double setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - setup0).count();
+  std::signal(SIGTERM, on_signal);
+  std::signal(SIGINT, on_signal);
+  std::cerr << "MGBFS_GRAPH_READY device=" << device << std::endl;
+  std::vector<double> times;
+  std::string status = "COMPLETE", reason = "FRONTIER_EXHAUSTED";
+  const auto t0 = std::chrono::steady_clock::now();
+  while (true) {
+    if (g_cancel) {
+      status = "INCOMPLETE";
+      reason = "CANCELLED";
+      break;
+    }
+    if (seconds && std::chrono::steady_clock::now() - t0 >= std::chrono::seconds(seconds)) {
+      status = "INCOMPLETE";
+      reason = "DEADLINE";
+      break;
+    }
+    const auto l0 = std::chrono::steady_clock::now();
+    uint64_t next = 0;
+    uint32_t fatal = 0;
+    const int s = mgbfs_engine_advance(engine, &next, &fatal);
+    if (s < 0) {
+      mgbfs_engine_destroy(engine);
+      throw CliError(std::string("ENGINE_ADVANCE: ") + mgbfs_engine_backend_name(nullptr));
+    }
+    if (s == MGBFS_LAYER) {
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count());
+      continue;
+    }
+    if (s == MGBFS_RESOURCE) {
+      status = "INCOMPLETE";
+      reason = "RESOURCE_" + std::to_string(fatal);
+    }
+    break;
+  }
+  const double bfs_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  uint32_t nl = 0;
+  mgbfs_engine_layer_sizes(engine, nullptr, 0, &nl);
+  std::vector<uint64_t> sizes(nl);
+  mgbfs_engine_layer_sizes(engine, sizes.data(), nl, &nl);
+  mgbfs_engine_info info{};
+  mgbfs_engine_query(engine, &info);
+  // Samples: <= 1000 rows of the last completed layer; the previous layer only if < 1000.
+  std::vector<int64_t> cur(1000 * static_cast<size_t>(graph.elements)), prev;
+  uint64_t ncur = 0, nprev = 0;
+  if (mgbfs_engine_sample(engine, 0, 1000, cur.data(), &ncur)) ncur = 0;
+  if (nl >= 2 && sizes[nl - 2] < 1000) {
+    prev.resize(1000 * static_cast<size_t>(graph.elements));
+    if (mgbfs_engine_sample(engine, 1, 1000, prev.data(), &nprev)) nprev = 0;
+  }
+  Json states = Json::object();
+  states["schema"] = Json::integer(2);
+  states["state_encoding"] = Json::str("signed_int64_vectors");
+  states["current"] = rows_json(cur, ncur, graph.elements);
+  states["previous_small"] = rows_json(prev, nprev, graph.elements);
+  states["current_sample_limit"] = Json::integer(1000);
+  const std::string states_text = mgbfs::host::dumps(states);
+
+  Json report = Json::object();
+  report["schema"] = Json::integer(2);
+  report["status"] = Json::str(status);
+  report["reason"] = Json::str(reason);
+  report["backend"] = Json::str(mgbfs_engine_backend_name(engine));
+  report["device"] = Json::integer(device);
+  report["world"] = Json::integer(1);
+  report["graph_digest"] = Json::str(graph.digest());
+  Json ls = Json::array();
+  for (uint64_t v : sizes) ls.push(Json::unsigned_(v));
+  report["layer_sizes"] = ls;
+  Json lt = Json::array();
+  for (double v : times) lt.push(Json::real(v));
+  report["layer_seconds"] = lt;
+  report["setup_seconds"] = Json::real(setup_seconds);
+  report["bfs_seconds"] = Json::real(bfs_seconds);
+  report["capacity"] = Json::unsigned_(capacity);
+  report["arena_bytes"] = Json::unsigned_(info.device_bytes);
+  Json plan = Json::object();
+  plan["state_bytes"] = Json::integer(info.state_bytes);
+  plan["elements"] = Json::integer(graph.elements);
+  plan["history_layers"] = Json::integer(info.history_layers);
+  plan["world"] = Json::integer(1);
+  plan["device_bytes"] = Json::unsigned_(info.device_bytes);
+  report["plan"] = plan;
+  Json devices = Json::array();
+  devices.push(Json::integer(device));
+  report["devices"] = devices;
+  report["state_bytes"] = Json::integer(info.state_bytes);
+  report["states_sha256"] = Json::str(mgbfs::host::sha256_hex(states_text));
+  report["edges_expanded"] = Json::unsigned_(info.candidates);
+  report["kernel_seconds"] = Json::real(info.kernel_seconds);
+  report["retention"] = Json::str("current sample <=1000 and previous layer only if <1000");
+  report["scope"] = Json::str("single-device MI355X engine");
+  mgbfs_engine_destroy(engine);
+  write_file(out + "/states.json", states_text);
+  write_file(out + "/report.json", mgbfs::host::dumps(report));
+  std::cout << mgbfs::host::dumps(report) << std::endl;
+  return 0;
+}
+
+int cmd_graph_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  std::cout << n << std::endl;
+  return 0;
+}
+
+int cmd_graph_info(const std::vector<std::string>& args) {
+  if (args.size() < 2) throw CliError("CLI_USAGE: mgbfs graph-info <graph.json> <devices> [shards] [capacity] [batch]");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  std::vector<int32_t> devs;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (args[1] == "auto") {
+    for (int d = 0; d < n; ++d) devs.push_back(d);
+  } else {
+    std::stringstream ss(args[1]);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) devs.push_back(static_cast<int32_t>(parse_u64(tok, "CLI_GRAPH_DEVICE")));
+  }
+  if (devs.empty()) throw CliError("GRAPH_INFO_NO_DEVICES");
+  Json out = Json::object();
+  out["graph_digest"] = Json::str(graph.digest());
+  Json dj = Json::array(), inv = Json::array();
+  for (int32_t d : devs) {
+    dj.push(Json::integer(d));
+    uint32_t cu = 0, ws = 0, lds = 0;
+    uint64_t tot = 0, fr = 0;
+    if (mgbfs_hip_device_info(d, &cu, &ws, &lds, &tot, &fr)) throw CliError("GRAPH_INFO_DEVICE");
+    Json e = Json::object();
+    e["device"] = Json::integer(d);
+    e["compute_units"] = Json::integer(cu);
+    e["wave_size"] = Json::integer(ws);
+    e["total_bytes"] = Json::unsigned_(tot);
+    e["free_bytes"] = Json::unsigned_(fr);
+    inv.push(e);
+  }
+  out["devices"] = dj;
+  out["inventory"] = inv;
+  Json plan = Json::object();
+  const bool packable = graph.kind == 0 && graph.elements <= 16 &&
+                        std::all_of(graph.start.begin(), graph.start.end(), [](int64_t v) { return v >= 0 && v < 16; });
+  const bool closed = graph.inverse_closed();
+  plan["inverse_closed"] = Json::boolean(closed);
+  plan["history_layers"] = Json::integer(closed ? 3 : 0);
+  plan["packed_u64"] = Json::boolean(packable);
+  plan["backend"] = Json::str(packable && closed ? "sorted" : (packable ? "permutation_bitmap" : "generic"));
+  plan["elements"] = Json::integer(graph.elements);
+  out["plan"] = plan;
+  out["additional_reserve_bytes"] = Json::unsigned_(2ull << 30);
+  std::cout << mgbfs::host::dumps(out) << std::endl;
+  return 0;
+}
+
+// Graph contract without a device: digest, validation, retention rule.
+int cmd_graph_digest(const std::vector<std::string>& args) {
+  if (args.size() != 1) throw CliError("CLI_USAGE: mgbfs graph-digest <graph.json>");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  Json out = Json::object();
+  out["graph_digest"] = Json::str(graph.digest());
+  out["inverse_closed"] = Json::boolean(graph.inverse_closed());
+  out["elements"] = Json::integer(graph.elements);
+  out["generators"] = Json::integer(graph.kind == 0 ? graph.perms.size() : graph.mats.size());
+  std::cout << mgbfs::host::dumps(out) << std::endl;
+  return 0;
+}
+
+int cmd_key_info() {
+  Json out = Json::object();
+  out["schema"] = Json::integer(1);
+  out["abi_version"] = Json::integer(mgbfs_hip_abi_version());
+  Json b = Json::array();
+  for (const char* x : {"sorted", "permutation_bitmap", "generic"}) b.push(Json::str(x));
+  out["backends"] = b;
+  Json c = Json::array();
+  for (const char* x : {"graph", "graph-info", "graph-count", "graph-digest", "key-info", "version"})
+    c.push(Json::str(x));
+  out["commands"] = c;
+  out["max_packed_degree"] = Json::integer(16);
+  out["state_encoding"] = Json::str("signed_int64_vectors");
+  out["device_arch"] = Json::str("gfx950");
+  std::cout << mgbfs::host::dumps(out) << std::endl;
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::vector<std::string> args(argv + 1, argv + argc);
+  try {
+    if (args.empty()) throw CliError("CLI_USAGE: mgbfs <graph|graph-info|graph-count|key-info|version> ...");
+    const std::string cmd = args[0];
+    std::vector<std::string> rest(args.begin() + 1, args.end());
+    if (cmd == "graph") return cmd_graph(rest);
+    if (cmd == "graph-info") return cmd_graph_info(rest);
+    if (cmd == "graph-count") return cmd_graph_count();
+    if (cmd == "graph-digest") return cmd_graph_digest(rest);
+    if (cmd == "key-info") return cmd_key_info();
+    if (cmd == "version") {
+      std::cout << "mgbfs-hip abi " << mgbfs_hip_abi_version() << std::endl;
+      return 0;
+    }
+    throw CliError("CLI_UNKNOWN_COMMAND");
+  } catch (const std::exception& e) {
+    std::string code = e.what();
+    Json err = Json::object();
+    err["status"] = Json::str("ERROR");
+    err["error"] = Json::str(code);
+    std::cerr << mgbfs::host::dumps(err) << std::endl;
+    return 1;
+  }
+}

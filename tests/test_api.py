@@ -1,0 +1,253 @@
+"""Host-side API contracts on the CPU: graph schema, digest, retention rule,
+the CayleyPy dispatch layer (with a stand-in CayleyGraph class, as the
+reference's tests use duck types) and the command-line host's validation
+paths that run before any device is touched."""
+import hashlib
+import json
+import subprocess
+import sys
+import types
+from pathlib import Path
+from types import SimpleNamespace
+from unittest.mock import patch
+
+import pytest
+
+from multigpubfs_amd import GraphDefinition, from_cayleypy, lrx
+from multigpubfs_amd import cayleypy_backend as api
+from multigpubfs_amd._native import CLI_PATH
+
+# ------------------------------------------------------------- schema / digest
+
+
+def test_validation_error_codes():
+    # tests/test_graph_definition.py:20-25 of the reference
+    for gens, start in [([[0, 0]], [0, 1]), ([[0, 1]], [0]), ([[0, 1]], [0, 1 << 63])]:
+        with pytest.raises(ValueError):
+            GraphDefinition.permutation(gens, start)
+    with pytest.raises(ValueError, match="GRAPH_MATRIX_GENERATOR"):
+        GraphDefinition.matrix(2, 1, [([1], 3)], [1, 2])
+    with pytest.raises(ValueError, match="GRAPH_MATRIX_MODULUS"):
+        GraphDefinition.matrix(1, 1, [([1], 1)], [1])
+    with pytest.raises(ValueError, match="GRAPH_SCHEMA"):
+        GraphDefinition({"schema": 2})
+
+
+def test_digest_is_name_independent_and_order_sensitive():
+    g = GraphDefinition.permutation([[1, 0, 2], [0, 2, 1]], [0, 1, 2], name="a")
+    h = GraphDefinition.permutation([[1, 0, 2], [0, 2, 1]], [0, 1, 2], name="b")
+    assert g.digest() == h.digest()
+    assert g.inverse_closed()
+    assert g.digest() != GraphDefinition.permutation([[0, 2, 1], [1, 0, 2]], [0, 1, 2]).digest()
+    assert GraphDefinition.from_dict(json.loads(g.to_json())).digest() == g.digest()
+    core = {"action": g.action, "schema": 2, "start": [0, 1, 2]}
+    expected = hashlib.sha256(json.dumps(core, sort_keys=True, separators=(",", ":")).encode()).hexdigest()
+    assert g.digest() == expected
+
+
+def test_inverse_closure_rule():
+    assert not GraphDefinition.permutation([[1, 2, 3, 0], [1, 0, 2, 3]], [0, 0, 1, 2]).inverse_closed()
+    assert lrx(7).inverse_closed()
+    assert not GraphDefinition.matrix(2, 1, [([1, 1, 0, 1], 257)], [256, 2]).inverse_closed()
+    # wrapping can fake a modular inverse (test_graph_definition.py:41-45)
+    assert not GraphDefinition.matrix(1, 1, [([(1 << 63) - 1], 3)], [2]).inverse_closed()
+    for modulus in (0, 4):
+        assert GraphDefinition.matrix(1, 1, [([(1 << 63) - 1], modulus)], [2]).inverse_closed()
+    assert GraphDefinition.matrix(1, 1, [([100], 101)], [2]).inverse_closed()
+    p = 11
+    sl2 = [([1, 1, 0, 1], p), ([1, 0, 1, 1], p), ([1, p - 1, 0, 1], p), ([1, 0, p - 1, 1], p)]
+    assert GraphDefinition.matrix(2, 2, sl2, [1, 0, 0, 1]).inverse_closed()
+    assert not GraphDefinition.matrix(2, 2, sl2[:2], [1, 0, 0, 1]).inverse_closed()
+
+
+def test_cayleypy_adapter_and_scalar_guards():
+    x = SimpleNamespace(generators_type=SimpleNamespace(name="PERMUTATION"), generators_permutations=[[1, 0]],
+                        central_state=[0, 0], name="coset", generator_names=["x"])
+    g = from_cayleypy(x)
+    assert g.start == [0, 0] and g.generator_names == ["x"]
+    x = SimpleNamespace(generators_type=SimpleNamespace(name="MATRIX"),
+                        generators_matrices=[SimpleNamespace(matrix=[[1, 1], [0, 1]], modulo=257)],
+                        central_state=[1, 2], name="rect", generator_names=["t"])
+    assert from_cayleypy(x).action["generators"] == [{"matrix": [1, 1, 0, 1], "modulo": 257}]
+    from multigpubfs_amd.graph_definition import _flatten
+    for bad in (0.5, True, "1", float("inf"), float("nan")):
+        with pytest.raises(ValueError):
+            _flatten(bad)
+    assert _flatten([1.0, -2, 3]) == [1, -2, 3]
+
+
+# --------------------------------------------------------- CayleyPy dispatch
+
+class FakeCayleyGraph:
+    """Stand-in for cayleypy.CayleyGraph: the dispatch layer only needs the
+    class attribute `bfs` and a `definition` with CayleyPy's attribute names."""
+
+    def __init__(self, n=3):
+        self.definition = SimpleNamespace(generators_type=SimpleNamespace(name="PERMUTATION"),
+                                          generators_permutations=lrx(n).action["generators"],
+                                          central_state=list(range(n)), name=f"lrx-{n}",
+                                          generator_names=["L", "R", "X"])
+
+    def bfs(self, **kwargs):
+        return ("torch", kwargs)
+
+
+@pytest.fixture
+def fake_cayleypy(monkeypatch, tmp_path):
+    module = types.ModuleType("cayleypy")
+    module.CayleyGraph = FakeCayleyGraph
+    monkeypatch.setitem(sys.modules, "cayleypy", module)
+    original = FakeCayleyGraph.bfs
+    yield SimpleNamespace(root=tmp_path, original=original)
+    try:
+        api.disable_native()
+    finally:
+        FakeCayleyGraph.bfs = original
+
+
+def fake_native(calls):
+    def native(definition, output, **kwargs):
+        calls.append((definition, kwargs))
+        output.mkdir(parents=True)
+        (output / "states.json").write_text(json.dumps(dict(current=[[2, 1, 0]], previous_small=[[0, 2, 1]])))
+        return dict(status="COMPLETE", layer_sizes=[1, 2, 3])
+    return native
+
+
+def test_simple_call_and_compact_samples(fake_cayleypy):
+    calls = []
+    with patch("multigpubfs_amd.launch.native_runtime", lambda *a: Path("/bin/true")):
+        api.enable_native(api.NativeOptions(output_root=fake_cayleypy.root))
+    with patch.object(api, "run_graph", side_effect=fake_native(calls)):
+        result = FakeCayleyGraph().bfs()
+    assert result.bfs_completed and result.layer_sizes == [1, 2, 3]
+    assert result.diameter() == 2 and result.num_vertices() == 6
+    assert set(result.layer_samples) == {1, 2} and result.layers == {}
+    assert calls[0][1]["devices"] is None and calls[0][1]["autotune"] is True
+    with pytest.raises(KeyError):
+        result.last_layer()
+
+
+def test_disable_restores_and_one_shot_does_not_patch(fake_cayleypy):
+    with patch("multigpubfs_amd.launch.native_runtime", lambda *a: Path("/bin/true")):
+        api.enable_native()
+        api.enable_native()
+    api.disable_native()
+    assert FakeCayleyGraph.bfs is fake_cayleypy.original
+    calls = []
+    with patch.object(api, "run_graph", side_effect=fake_native(calls)):
+        api.bfs(FakeCayleyGraph(), output=fake_cayleypy.root / "one")
+    assert FakeCayleyGraph.bfs is fake_cayleypy.original and len(calls) == 1
+
+
+def test_foreign_replacement_is_detected(fake_cayleypy):
+    with patch("multigpubfs_amd.launch.native_runtime", lambda *a: Path("/bin/true")):
+        api.enable_native()
+    FakeCayleyGraph.bfs = lambda self, **kw: "other"
+    with pytest.raises(RuntimeError):
+        api.disable_native()
+    FakeCayleyGraph.bfs = api._INSTALLED
+    api.disable_native()
+
+
+def test_torch_backend_preserves_kwargs_and_unsupported_raise(fake_cayleypy):
+    with patch("multigpubfs_amd.launch.native_runtime", lambda *a: Path("/bin/true")):
+        api.enable_native()
+    assert FakeCayleyGraph().bfs(backend="torch", max_diameter=2) == ("torch", dict(max_diameter=2))
+    for kwargs in (dict(return_all_edges=True), dict(max_diameter=2), dict(max_layer_size_to_store=None),
+                   dict(misspelled=True)):
+        with patch.object(api, "run_graph") as run:
+            with pytest.raises(NotImplementedError):
+                FakeCayleyGraph().bfs(**kwargs)
+            run.assert_not_called()
+    # auto falls back to the original for unsupported arguments
+    assert FakeCayleyGraph().bfs(backend="auto", max_diameter=2)[0] == "torch"
+
+
+def test_single_decoded_start_and_multisource(fake_cayleypy):
+    calls = []
+    with patch.object(api, "run_graph", side_effect=fake_native(calls)):
+        api.bfs(FakeCayleyGraph(), output=fake_cayleypy.root / "start", start_states=[[2, 1, 0]])
+    assert calls[0][0].start == [2, 1, 0]
+    with pytest.raises(NotImplementedError):
+        api.bfs(FakeCayleyGraph(), start_states=[[0, 1, 2], [2, 1, 0]])
+
+
+def test_incomplete_has_no_diameter(fake_cayleypy):
+    def incomplete(definition, output, **kwargs):
+        report = fake_native([])(definition, output, **kwargs)
+        report["status"] = "INCOMPLETE"
+        return report
+    with patch.object(api, "run_graph", side_effect=incomplete):
+        result = api.bfs(FakeCayleyGraph(), output=fake_cayleypy.root / "inc")
+    assert not result.bfs_completed and result.completed_depth == 2
+    with pytest.raises(RuntimeError):
+        result.diameter()
+
+
+def test_options_validation():
+    with pytest.raises(ValueError):
+        api.NativeOptions(memory_preset="fast")
+    with pytest.raises(ValueError):
+        api.NativeOptions(segment_rows=0)
+    with pytest.raises(ValueError):
+        api.NativeOptions(memory_backend="tape")
+
+
+def test_run_graph_argument_validation(tmp_path):
+    from multigpubfs_amd import run_graph
+    g = lrx(4)
+    for kwargs, code in [(dict(device=0, devices=[0]), "DEVICE_SELECTION_CONFLICT"), (dict(device=-1), "INVALID_DEVICE"),
+                         (dict(devices=[0, 0]), "INVALID_DEVICES"), (dict(devices=[0, 1]), "SINGLE_DEVICE_BUILD"),
+                         (dict(capacity=0), "INVALID_CAPACITY"), (dict(backend="cuda"), "INVALID_BACKEND")]:
+        with pytest.raises(ValueError, match=code):
+            run_graph(g, tmp_path / "out", **kwargs)
+    m = GraphDefinition.matrix(1, 1, [([2], 5)], [1])
+    with pytest.raises(ValueError, match="BITMAP_REQUIRES_PERMUTATION_ACTION"):
+        run_graph(m, tmp_path / "out", memory_backend="permutation_bitmap")
+
+
+# ------------------------------------------------------ command-line host
+
+def cli(*args):
+    return subprocess.run([str(CLI_PATH), *map(str, args)], capture_output=True, text=True)
+
+
+def last_error(proc):
+    return json.loads(proc.stderr.strip().splitlines()[-1])
+
+
+def test_cli_digest_matches_python_and_retention_rule(tmp_path):
+    graphs = [lrx(13), GraphDefinition.permutation([[1, 2, 3, 0], [1, 0, 2, 3]], [0, 0, 1, 2], name="é"),
+              GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], 7), ([1, 6, 0, 1], 7)], [1, 0, 0, 1]),
+              GraphDefinition.matrix(1, 1, [([(1 << 63) - 1], 3)], [-5])]
+    for i, g in enumerate(graphs):
+        path = tmp_path / f"g{i}.json"
+        path.write_text(g.to_json())
+        proc = cli("graph-digest", path)
+        assert proc.returncode == 0, proc.stderr
+        doc = json.loads(proc.stdout)
+        assert doc["graph_digest"] == g.digest()
+        assert doc["inverse_closed"] == g.inverse_closed()
+
+
+def test_cli_rejects_invalid_graphs_and_existing_output(tmp_path):
+    bad = tmp_path / "bad.json"
+    for doc, code in [({"schema": 2}, "GRAPH_SCHEMA"),
+                      (dict(json.loads(lrx(3).to_json()), start=[0, 1]), "GRAPH_PERMUTATION_SHAPE")]:
+        bad.write_text(json.dumps(doc))
+        proc = cli("graph", bad, tmp_path / "o1")
+        assert proc.returncode != 0 and last_error(proc) == {"status": "ERROR", "error": code}
+    good = tmp_path / "good.json"
+    good.write_text(lrx(3).to_json())
+    (tmp_path / "exists").mkdir()
+    proc = cli("graph", good, tmp_path / "exists")
+    assert last_error(proc)["error"] == "CLI_GRAPH_OUTPUT_EXISTS"
+    assert last_error(cli("graph", good, tmp_path / "o2", "--bogus", "1"))["error"] == "CLI_GRAPH_UNKNOWN_OPTION"
+    assert last_error(cli("frobnicate"))["error"] == "CLI_UNKNOWN_COMMAND"
+
+
+def test_cli_key_info():
+    proc = cli("key-info")
+    doc = json.loads(proc.stdout)
+    assert proc.returncode == 0 and "graph" in doc["commands"] and doc["device_arch"] == "gfx950"
