@@ -146,6 +146,47 @@ int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint3
                                uint64_t* next, uint64_t* next_count, uint64_t next_capacity, uint32_t* error,
                                void* stream);
 
+/* ------------------------------------------------- batched instances */
+/* Many independent BFS instances advanced together, one layer of every
+ * active instance per step, by a hipGraph-captured layer loop; instances are
+ * admitted and released at layer boundaries (continuous batching).  The
+ * reference has no batched kernel (its nearest relative is the sequential
+ * resident worker of crates/mgbfs-runtime/src/generic_session.rs:6-71 and
+ * multigpubfs/generic_session.py:56-144); parity is each instance's layer
+ * table, identical to the one-at-a-time engine's. */
+#define MGBFS_BATCH_FULL 3 /* no free slot or the planned VRAM exceeds the budget */
+
+typedef struct mgbfs_batch_options {
+  int32_t device;
+  uint32_t max_slots;        /* <= 1024 (0 = 256)                              */
+  uint64_t vram_budget;      /* bytes for admitted jobs (0 = free - 4 GiB)      */
+  uint32_t layers_per_graph; /* layers captured per hipGraph launch (0 = 4)     */
+  uint32_t grid;             /* persistent expand grid (0 = 8 x CUs)            */
+} mgbfs_batch_options;
+
+int mgbfs_batch_create(const mgbfs_batch_options* options, void** batch, char* error_text, size_t error_capacity);
+/* Planned device bytes of a job; visited 0 = autotuned, 1 = bitmap, 2 = hash.
+ * *chosen_visited receives 1 (Lehmer-rank bitmap) or 2 (open-addressing hash). */
+int mgbfs_batch_plan(void* batch, const mgbfs_graph_desc* graph, uint32_t visited, uint32_t max_layers,
+                     uint64_t* bytes, uint32_t* chosen_visited);
+/* Admits a job into a free slot (returns MGBFS_BATCH_FULL when none fits). */
+int mgbfs_batch_admit(void* batch, const mgbfs_graph_desc* graph, uint32_t visited, uint32_t max_layers,
+                      int32_t* slot);
+/* Runs up to `layers` layers of all active instances; *active = still active. */
+int mgbfs_batch_step(void* batch, uint32_t layers, uint32_t* active);
+/* Steps until no instance is active. */
+int mgbfs_batch_run(void* batch);
+/* Slots that finished since the last poll. */
+int mgbfs_batch_poll(void* batch, int32_t* slots, uint32_t capacity, uint32_t* count);
+/* Layer table of a slot (depth 0 first); status 2 = done; error = sticky bits. */
+int mgbfs_batch_result(void* batch, int32_t slot, uint64_t* sizes, uint32_t capacity, uint32_t* count,
+                       uint32_t* status, uint32_t* error);
+/* Current-layer states of a slot as int64 rows. */
+int mgbfs_batch_sample(void* batch, int32_t slot, uint64_t limit, int64_t* rows, uint64_t* count);
+int mgbfs_batch_release(void* batch, int32_t slot);
+int mgbfs_batch_usage(void* batch, uint64_t* used, uint64_t* budget, uint32_t* slots);
+void mgbfs_batch_destroy(void* batch);
+
 /* --------------------------------------------- generic SoA kernel ABI */
 /* Same signatures and argument meaning as the reference's libmgbfs_cuda
  * (cuda/generic_visited.h:3-11 and :22-44, cuda/generic_graph.h:3-6):

@@ -230,3 +230,145 @@ def device_info(device: int = 0) -> dict:
         raise RuntimeError(f"HIP_DEVICE_INFO_{rc}")
     return dict(compute_units=cu.value, wave_size=ws.value, lds_bytes=lds.value, total_bytes=tot.value,
                 free_bytes=fr.value)
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_slots", C.c_uint32), ("vram_budget", C.c_uint64),
+                ("layers_per_graph", C.c_uint32), ("grid", C.c_uint32)]
+
+
+MGBFS_BATCH_FULL = 3
+VISITED = {"auto": 0, "bitmap": 1, "hash": 2}
+
+
+def _bind_batch(lib):
+    if getattr(lib, "_batch_bound", False):
+        return lib
+    p, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+    lib.mgbfs_batch_create.argtypes = [C.POINTER(BatchOptions), C.POINTER(p), C.c_char_p, C.c_size_t]
+    lib.mgbfs_batch_plan.argtypes = [p, C.POINTER(GraphDesc), u32, u32, C.POINTER(u64), C.POINTER(u32)]
+    lib.mgbfs_batch_admit.argtypes = [p, C.POINTER(GraphDesc), u32, u32, C.POINTER(i32)]
+    lib.mgbfs_batch_step.argtypes = [p, u32, C.POINTER(u32)]
+    lib.mgbfs_batch_run.argtypes = [p]
+    lib.mgbfs_batch_poll.argtypes = [p, C.POINTER(i32), u32, C.POINTER(u32)]
+    lib.mgbfs_batch_result.argtypes = [p, i32, C.POINTER(u64), u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    lib.mgbfs_batch_sample.argtypes = [p, i32, u64, C.POINTER(C.c_int64), C.POINTER(u64)]
+    lib.mgbfs_batch_release.argtypes = [p, i32]
+    lib.mgbfs_batch_usage.argtypes = [p, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]
+    lib.mgbfs_batch_destroy.argtypes = [p]
+    lib.mgbfs_batch_destroy.restype = None
+    for name in ("create", "plan", "admit", "step", "run", "poll", "result", "sample", "release", "usage"):
+        getattr(lib, "mgbfs_batch_" + name).restype = C.c_int
+    lib._batch_bound = True
+    return lib
+
+
+def graph_desc(graph):
+    """mgbfs_graph_desc for a GraphDefinition (arrays kept alive on the object)."""
+    from .graph_definition import GraphDefinition
+    gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
+    data = gd.to_dict()
+    a = data["action"]
+    desc = GraphDesc()
+    desc.elements = len(data["start"])
+    desc.generators = len(a["generators"])
+    desc.inverse_closed = int(gd.inverse_closed())
+    desc.expected_states = data["expected_max_unique_states"] or 0
+    keep = [_array(C.c_int64, data["start"])]
+    desc.start = keep[0]
+    if a["kind"] == "permutation":
+        desc.kind, desc.rows, desc.cols = 0, desc.elements, 1
+        keep.append(_array(C.c_uint32, [v for g in a["generators"] for v in g]))
+        desc.permutations = keep[-1]
+    else:
+        desc.kind, desc.rows, desc.cols = 1, a["rows"], a["cols"]
+        keep.append(_array(C.c_int64, [v for g in a["generators"] for v in g["matrix"]]))
+        desc.matrices = keep[-1]
+        keep.append(_array(C.c_uint32, [g["modulo"] for g in a["generators"]]))
+        desc.moduli = keep[-1]
+    desc._keep = keep
+    return desc
+
+
+class BatchEngine:
+    """Many BFS instances on one GPU, advanced together (mgbfs_batch_*)."""
+
+    def __init__(self, *, device=0, max_slots=256, vram_budget=0, layers_per_graph=0, grid=0):
+        self._lib = _bind_batch(load())
+        opts = BatchOptions(device, max_slots, vram_budget, layers_per_graph, grid)
+        h = C.c_void_p()
+        err = C.create_string_buffer(256)
+        if self._lib.mgbfs_batch_create(C.byref(opts), C.byref(h), err, 256):
+            raise RuntimeError(err.value.decode() or "BATCH_CREATE")
+        self._h = h
+        self.max_slots = max_slots
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mgbfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise RuntimeError(f"{what}: {self._lib.mgbfs_engine_backend_name(None).decode()}")
+        return rc
+
+    def plan(self, graph, visited="auto", max_layers=0):
+        nbytes, chosen = C.c_uint64(), C.c_uint32()
+        self._check(self._lib.mgbfs_batch_plan(self._h, C.byref(graph_desc(graph)), VISITED[visited], max_layers,
+                                               C.byref(nbytes), C.byref(chosen)), "BATCH_PLAN")
+        return nbytes.value, {1: "bitmap", 2: "hash"}[chosen.value]
+
+    def admit(self, graph, visited="auto", max_layers=0):
+        """Slot index, or None when no slot or VRAM budget is free."""
+        slot = C.c_int32()
+        rc = self._check(self._lib.mgbfs_batch_admit(self._h, C.byref(graph_desc(graph)), VISITED[visited],
+                                                     max_layers, C.byref(slot)), "BATCH_ADMIT")
+        return None if rc == MGBFS_BATCH_FULL else slot.value
+
+    def step(self, layers=4):
+        active = C.c_uint32()
+        self._check(self._lib.mgbfs_batch_step(self._h, layers, C.byref(active)), "BATCH_STEP")
+        return active.value
+
+    def run(self):
+        self._check(self._lib.mgbfs_batch_run(self._h), "BATCH_RUN")
+
+    def poll(self):
+        buf = (C.c_int32 * self.max_slots)()
+        n = C.c_uint32()
+        self._check(self._lib.mgbfs_batch_poll(self._h, buf, self.max_slots, C.byref(n)), "BATCH_POLL")
+        return list(buf[:n.value])
+
+    def result(self, slot):
+        n, status, error = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.mgbfs_batch_result(self._h, slot, None, 0, C.byref(n), C.byref(status),
+                                                 C.byref(error)), "BATCH_RESULT")
+        buf = (C.c_uint64 * max(1, n.value))()
+        self._check(self._lib.mgbfs_batch_result(self._h, slot, buf, n.value, C.byref(n), C.byref(status),
+                                                 C.byref(error)), "BATCH_RESULT")
+        return list(buf[:n.value]), status.value, error.value
+
+    def sample(self, slot, width, limit=1000):
+        rows = (C.c_int64 * max(1, limit * width))()
+        n = C.c_uint64()
+        self._check(self._lib.mgbfs_batch_sample(self._h, slot, limit, rows, C.byref(n)), "BATCH_SAMPLE")
+        flat = rows[:n.value * width]
+        return [list(flat[i * width:(i + 1) * width]) for i in range(n.value)]
+
+    def release(self, slot):
+        self._check(self._lib.mgbfs_batch_release(self._h, slot), "BATCH_RELEASE")
+
+    def usage(self):
+        used, budget, slots = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._lib.mgbfs_batch_usage(self._h, C.byref(used), C.byref(budget), C.byref(slots))
+        return dict(used=used.value, budget=budget.value, slots=slots.value)

@@ -1,0 +1,58 @@
+// Batched multi-instance BFS: device-resident instance table shared by the
+// batch kernels and the batch engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "common.hpp"
+
+namespace mgbfs {
+
+constexpr uint32_t kBatchMaxSlots = 1024;   // instances per batch (one scan block)
+constexpr uint32_t kBatchMaxGens = 16;
+constexpr uint32_t kBatchMaxEntries = 9;    // matrix states: rows*cols <= 9 packed entries
+
+enum BatchAction : uint32_t { kActPerm = 0, kActMatrix = 1 };
+enum BatchVisited : uint32_t { kVisBitmap = 0, kVisHash = 1 };
+
+// Instance status bits.
+constexpr uint32_t kInstActive = 1u;
+constexpr uint32_t kInstDone = 2u;
+
+struct BatchInstance {
+  // Control (device-updated by the finish kernel).
+  uint32_t status;        // kInstActive / kInstDone
+  uint32_t depth;         // index of the current layer
+  uint32_t error;         // sticky error bits (kErr*)
+  uint32_t pad0;
+  uint64_t count;         // current layer size
+  unsigned long long next_count;
+  uint64_t* front;        // current layer keys
+  uint64_t* next;         // next layer keys
+  uint64_t cap;           // capacity of front/next
+  uint64_t* layer_sizes;  // [max_layers]
+  uint32_t max_layers;
+  // Visited set.
+  uint32_t visited;       // kVisBitmap / kVisHash
+  uint32_t* bitmap;       // kVisBitmap: n! bits (Lehmer rank)
+  uint64_t bitmap_bits;
+  uint64_t* table;        // kVisHash: open addressing, EMPTY = ~0, key in slot
+  uint64_t table_mask;
+  // Action.
+  uint32_t action;        // kActPerm / kActMatrix
+  uint32_t degree;        // permutation degree n
+  uint32_t generators;
+  uint32_t rows, cols;    // matrix action: rows x cols state, rows x rows generators
+  uint32_t ebits;         // matrix: bits per packed entry
+  int64_t modulus;        // matrix: common modulus (> 0)
+  uint8_t src[kBatchMaxGens][kMaxDegree];         // permutation tables
+  int32_t mat[kBatchMaxGens][kBatchMaxEntries];   // matrix generators (row-major rows x rows)
+};
+
+hipError_t batch_reset_flag(uint32_t* flag, hipStream_t stream);
+hipError_t batch_tiles(BatchInstance* inst, uint32_t slots, uint32_t tile, uint32_t* prefix, hipStream_t stream);
+hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t tile, uint32_t grid,
+                        uint32_t threads, hipStream_t stream);
+hipError_t batch_finish(BatchInstance* inst, uint32_t slots, uint32_t* any_active, hipStream_t stream);
+hipError_t batch_seed(BatchInstance* inst, uint32_t slot, uint64_t start_key, hipStream_t stream);
+
+}  // namespace mgbfs

@@ -1,0 +1,459 @@
+// Host side of the batched multi-instance engine: slot admission with VRAM
+// planning, a pooled device allocator, and a hipGraph-captured layer loop
+// (reset flag, then K x [tiles, expand, finish]) replayed until no instance
+// is active.  Instances join and leave at layer boundaries, so the same
+// object serves both the fixed sweep and continuous-batching serving.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+#include "batch.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace mgbfs {
+
+namespace {
+
+constexpr uint32_t kLayersPerGraph = 4;
+constexpr uint32_t kDefaultMaxLayers = 4096;
+constexpr uint32_t kTile = 512;
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Reuses device allocations across admitted jobs (exact byte size match).
+class BufferPool {
+ public:
+  DeviceBuffer take(uint64_t bytes) {
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+      DeviceBuffer b = std::move(it->second);
+      free_.erase(it);
+      pooled_ -= bytes;
+      return b;
+    }
+    DeviceBuffer b;
+    b.allocate(bytes);
+    return b;
+  }
+  void give(DeviceBuffer&& b) {
+    if (!b.bytes()) return;
+    pooled_ += b.bytes();
+    free_.emplace(b.bytes(), std::move(b));
+  }
+  void trim() {
+    free_.clear();
+    pooled_ = 0;
+  }
+  uint64_t pooled() const { return pooled_; }
+
+ private:
+  std::multimap<uint64_t, DeviceBuffer> free_;
+  uint64_t pooled_ = 0;
+};
+
+struct Slot {
+  bool used = false;
+  bool reported = false;
+  uint64_t planned = 0;
+  DeviceBuffer front, next, sizes, visited;
+};
+
+}  // namespace
+
+struct BatchPlan {
+  uint32_t visited = kVisBitmap;
+  uint64_t bound = 0, cap = 0, visited_bytes = 0, bytes = 0;
+  uint32_t max_layers = kDefaultMaxLayers;
+};
+
+class BatchEngine {
+ public:
+  BatchEngine(const mgbfs_batch_options& o) : device_(o.device) {
+    MGBFS_HIP(hipSetDevice(device_));
+    MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    slots_n_ = o.max_slots ? std::min(o.max_slots, kBatchMaxSlots) : 256;
+    budget_ = o.vram_budget;
+    if (!budget_) {
+      size_t fr = 0, tot = 0;
+      MGBFS_HIP(hipMemGetInfo(&fr, &tot));
+      budget_ = fr > (4ull << 30) ? fr - (4ull << 30) : 0;
+    }
+    hipDeviceProp_t prop;
+    MGBFS_HIP(hipGetDeviceProperties(&prop, device_));
+    grid_ = o.grid ? o.grid : static_cast<uint32_t>(prop.multiProcessorCount) * 8;
+    layers_per_graph_ = o.layers_per_graph ? o.layers_per_graph : kLayersPerGraph;
+    inst_.allocate(sizeof(BatchInstance) * slots_n_);
+    prefix_.allocate(4 * (slots_n_ + 1));
+    flag_.allocate(64);
+    MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_flag_), 64, hipHostMallocDefault));
+    host_inst_.assign(slots_n_, BatchInstance{});
+    std::memset(host_inst_.data(), 0, sizeof(BatchInstance) * slots_n_);
+    MGBFS_HIP(hipMemcpy(inst_.as<void>(), host_inst_.data(), sizeof(BatchInstance) * slots_n_, hipMemcpyHostToDevice));
+    slots_.resize(slots_n_);
+    capture();
+  }
+  ~BatchEngine() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (host_flag_) (void)hipHostFree(host_flag_);
+    for (auto& s : slots_) s = Slot{};
+    pool_.trim();
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  // Planned footprint (the admission unit) and the visited-set choice.
+  BatchPlan plan(const Graph& g, uint32_t visited, uint32_t max_layers) const {
+    BatchPlan p;
+    p.max_layers = max_layers ? max_layers : kDefaultMaxLayers;
+    if (g.kind == 0) {
+      if (!g.packable() || g.elements > 15) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+    } else {
+      if (g.elements > kBatchMaxEntries || g.rows > 3 || g.generators > kBatchMaxGens)
+        throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+      const uint32_t m = g.moduli[0];
+      for (uint32_t x : g.moduli)
+        if (x != m || m < 2) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+      for (int64_t v : g.start)
+        if (v < 0 || v >= static_cast<int64_t>(m)) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+      for (int64_t v : g.matrices)
+        if (v > INT32_MAX || v < INT32_MIN) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+      uint32_t eb = 1;
+      while ((1ull << eb) < m) ++eb;
+      if (eb * g.elements > 63) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+    }
+    if (g.generators > kBatchMaxGens) throw std::runtime_error("BATCH_UNSUPPORTED_GRAPH");
+    p.bound = state_space_bound(g);
+    if (!p.bound || p.bound > (1ull << 33)) throw std::runtime_error("BATCH_STATE_BOUND");
+    const bool bitmap_ok = g.kind == 0 && g.distinct_perm() && factorial_u64(g.elements) <= (1ull << 34);
+    const uint64_t bitmap_bytes = bitmap_ok ? (factorial_u64(g.elements) + 31) / 32 * 4 : ~0ull;
+    const uint64_t hash_bytes = next_pow2(2 * p.bound) * 8;
+    if (visited == 0) p.visited = bitmap_ok && bitmap_bytes <= hash_bytes ? kVisBitmap : kVisHash;
+    else p.visited = visited == 1 ? kVisBitmap : kVisHash;
+    if (p.visited == kVisBitmap && !bitmap_ok) throw std::runtime_error("BATCH_BITMAP_UNSUPPORTED");
+    p.visited_bytes = p.visited == kVisBitmap ? bitmap_bytes : hash_bytes;
+    p.cap = p.bound;
+    p.bytes = 2 * p.cap * 8 + p.visited_bytes + 8ull * p.max_layers;
+    return p;
+  }
+
+  int admit(const Graph& g, uint32_t visited, uint32_t max_layers, int32_t* slot_out) {
+    const BatchPlan p = plan(g, visited, max_layers);
+    if (used_ + p.bytes > budget_) return MGBFS_BATCH_FULL;
+    int32_t slot = -1;
+    for (uint32_t s = 0; s < slots_n_; ++s)
+      if (!slots_[s].used) {
+        slot = static_cast<int32_t>(s);
+        break;
+      }
+    if (slot < 0) return MGBFS_BATCH_FULL;
+    MGBFS_HIP(hipSetDevice(device_));
+    Slot& sl = slots_[slot];
+    sl.front = pool_.take(p.cap * 8);
+    sl.next = pool_.take(p.cap * 8);
+    sl.sizes = pool_.take(8ull * p.max_layers);
+    sl.visited = pool_.take(p.visited_bytes);
+    BatchInstance& d = host_inst_[slot];
+    std::memset(&d, 0, sizeof d);
+    d.front = sl.front.as<uint64_t>();
+    d.next = sl.next.as<uint64_t>();
+    d.cap = p.cap;
+    d.layer_sizes = sl.sizes.as<uint64_t>();
+    d.max_layers = p.max_layers;
+    d.visited = p.visited;
+    d.generators = g.generators;
+    uint64_t start_key = 0;
+    if (p.visited == kVisBitmap) {
+      d.bitmap = sl.visited.as<uint32_t>();
+      d.bitmap_bits = factorial_u64(g.elements);
+      MGBFS_HIP(fill_u32(d.bitmap, 0u, p.visited_bytes / 4, stream_));
+    } else {
+      d.table = sl.visited.as<uint64_t>();
+      d.table_mask = p.visited_bytes / 8 - 1;
+      MGBFS_HIP(fill_u64(d.table, ~0ull, p.visited_bytes / 8, stream_));
+    }
+    if (g.kind == 0) {
+      d.action = kActPerm;
+      d.degree = g.elements;
+      for (uint32_t k = 0; k < g.generators; ++k)
+        for (uint32_t i = 0; i < g.elements; ++i) d.src[k][i] = static_cast<uint8_t>(g.perms[k * g.elements + i]);
+      start_key = g.start_key();
+    } else {
+      d.action = kActMatrix;
+      d.rows = g.rows;
+      d.cols = g.cols;
+      d.modulus = g.moduli[0];
+      uint32_t eb = 1;
+      while ((1ull << eb) < static_cast<uint64_t>(d.modulus)) ++eb;
+      d.ebits = eb;
+      for (uint32_t k = 0; k < g.generators; ++k)
+        for (uint32_t e = 0; e < g.rows * g.rows; ++e)
+          d.mat[k][e] = static_cast<int32_t>(g.matrices[static_cast<size_t>(k) * g.rows * g.rows + e]);
+      for (uint32_t e = 0; e < g.elements; ++e) start_key |= static_cast<uint64_t>(g.start[e]) << (e * eb);
+    }
+    // The stream is idle between steps: descriptor upload, then seed.
+    MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
+    MGBFS_HIP(batch_seed(inst_.as<BatchInstance>(), static_cast<uint32_t>(slot), start_key, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    sl.used = true;
+    sl.reported = false;
+    sl.planned = p.bytes;
+    used_ += p.bytes;
+    kinds_[slot] = g.kind;
+    rows_[slot] = g.rows;
+    cols_[slot] = g.cols;
+    *slot_out = slot;
+    return 0;
+  }
+
+  // Runs up to `layers` layers of every active instance; returns how many
+  // instances are still active afterwards.
+  uint32_t step(uint32_t layers) {
+    MGBFS_HIP(hipSetDevice(device_));
+    const uint32_t launches = std::max<uint32_t>(1, (layers + layers_per_graph_ - 1) / layers_per_graph_);
+    for (uint32_t k = 0; k < launches; ++k) MGBFS_HIP(hipGraphLaunch(exec_, stream_));
+    refresh();
+    uint32_t active = 0;
+    for (uint32_t s = 0; s < slots_n_; ++s)
+      if (slots_[s].used && (host_inst_[s].status & kInstActive)) ++active;
+    return active;
+  }
+
+  void run_all() {
+    while (step(64)) {
+    }
+  }
+
+  // Slots whose instance finished and has not been reported yet.
+  std::vector<int32_t> poll() {
+    std::vector<int32_t> out;
+    for (uint32_t s = 0; s < slots_n_; ++s)
+      if (slots_[s].used && !slots_[s].reported && (host_inst_[s].status & kInstDone)) {
+        slots_[s].reported = true;
+        out.push_back(static_cast<int32_t>(s));
+      }
+    return out;
+  }
+
+  void result(int32_t slot, std::vector<uint64_t>& sizes, uint32_t& status, uint32_t& error) {
+    check_slot(slot);
+    const BatchInstance& d = host_inst_[slot];
+    sizes.assign(d.depth + 1, 0);
+    MGBFS_HIP(hipMemcpy(sizes.data(), d.layer_sizes, sizes.size() * 8, hipMemcpyDeviceToHost));
+    status = d.status;
+    error = d.error;
+  }
+
+  // Current-layer states of a slot as int64 rows (at most limit).
+  void sample(int32_t slot, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) {
+    check_slot(slot);
+    const BatchInstance& d = host_inst_[slot];
+    count = std::min<uint64_t>(limit, d.count);
+    std::vector<uint64_t> keys(count);
+    if (count) MGBFS_HIP(hipMemcpy(keys.data(), d.front, count * 8, hipMemcpyDeviceToHost));
+    const uint32_t w = d.action == kActPerm ? d.degree : d.rows * d.cols;
+    rows.assign(count * w, 0);
+    for (uint64_t r = 0; r < count; ++r)
+      for (uint32_t e = 0; e < w; ++e) {
+        const uint32_t bits = d.action == kActPerm ? 4 : d.ebits;
+        rows[r * w + e] = static_cast<int64_t>((keys[r] >> (e * bits)) & ((1ull << bits) - 1));
+      }
+  }
+
+  void release(int32_t slot) {
+    check_slot(slot);
+    MGBFS_HIP(hipSetDevice(device_));
+    Slot& sl = slots_[slot];
+    BatchInstance& d = host_inst_[slot];
+    std::memset(&d, 0, sizeof d);
+    MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    pool_.give(std::move(sl.front));
+    pool_.give(std::move(sl.next));
+    pool_.give(std::move(sl.sizes));
+    pool_.give(std::move(sl.visited));
+    used_ -= sl.planned;
+    sl = Slot{};
+    // Pooled buffers count against the budget only while they fit beside live jobs.
+    if (used_ + pool_.pooled() > budget_) pool_.trim();
+  }
+
+  uint64_t used() const { return used_; }
+  uint64_t budget() const { return budget_; }
+  uint32_t slots() const { return slots_n_; }
+
+ private:
+  void check_slot(int32_t slot) const {
+    if (slot < 0 || static_cast<uint32_t>(slot) >= slots_n_ || !slots_[slot].used)
+      throw std::runtime_error("BATCH_SLOT");
+  }
+  void capture() {
+    BatchInstance* inst = inst_.as<BatchInstance>();
+    uint32_t* prefix = prefix_.as<uint32_t>();
+    uint32_t* flag = flag_.as<uint32_t>();
+    MGBFS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    MGBFS_HIP(batch_reset_flag(flag, stream_));
+    for (uint32_t k = 0; k < layers_per_graph_; ++k) {
+      MGBFS_HIP(batch_tiles(inst, slots_n_, kTile, prefix, stream_));
+      MGBFS_HIP(batch_expand(inst, slots_n_, prefix, kTile, grid_, 256, stream_));
+      MGBFS_HIP(batch_finish(inst, slots_n_, flag, stream_));
+    }
+    MGBFS_HIP(hipStreamEndCapture(stream_, &graph_));
+    MGBFS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+  void refresh() {
+    MGBFS_HIP(hipMemcpyAsync(host_inst_.data(), inst_.as<void>(), sizeof(BatchInstance) * slots_n_,
+                             hipMemcpyDeviceToHost, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  uint32_t slots_n_ = 0, grid_ = 0, layers_per_graph_ = kLayersPerGraph;
+  uint64_t budget_ = 0, used_ = 0;
+  DeviceBuffer inst_, prefix_, flag_;
+  uint32_t* host_flag_ = nullptr;
+  std::vector<BatchInstance> host_inst_;
+  std::vector<Slot> slots_;
+  std::map<int32_t, uint32_t> kinds_, rows_, cols_;
+  BufferPool pool_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+};
+
+Graph intake_graph(const mgbfs_graph_desc* d);
+
+}  // namespace mgbfs
+
+using mgbfs::BatchEngine;
+
+#define MGBFS_BATCH_GUARD(...)                          \
+  try {                                                 \
+    __VA_ARGS__                                         \
+  } catch (const mgbfs::HipError& e) {                  \
+    mgbfs::set_last_error(e.what());                    \
+    return -static_cast<int>(e.code == 0 ? 1 : e.code); \
+  } catch (const std::exception& e) {                   \
+    mgbfs::set_last_error(e.what());                    \
+    return -1;                                          \
+  }
+
+extern "C" int mgbfs_batch_create(const mgbfs_batch_options* options, void** batch, char* error_text,
+                                  size_t error_capacity) {
+  if (!options || !batch) return hipErrorInvalidValue;
+  *batch = nullptr;
+  try {
+    *batch = new BatchEngine(*options);
+    return 0;
+  } catch (const std::exception& e) {
+    if (error_text && error_capacity) {
+      std::strncpy(error_text, e.what(), error_capacity - 1);
+      error_text[error_capacity - 1] = 0;
+    }
+    mgbfs::set_last_error(e.what());
+    return -1;
+  }
+}
+
+extern "C" int mgbfs_batch_plan(void* batch, const mgbfs_graph_desc* graph, uint32_t visited, uint32_t max_layers,
+                                uint64_t* bytes, uint32_t* chosen_visited) {
+  if (!batch || !graph) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    const mgbfs::BatchPlan p =
+        static_cast<BatchEngine*>(batch)->plan(mgbfs::intake_graph(graph), visited, max_layers);
+    if (bytes) *bytes = p.bytes;
+    if (chosen_visited) *chosen_visited = p.visited == mgbfs::kVisBitmap ? 1u : 2u;
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_admit(void* batch, const mgbfs_graph_desc* graph, uint32_t visited, uint32_t max_layers,
+                                 int32_t* slot) {
+  if (!batch || !graph || !slot) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    return static_cast<BatchEngine*>(batch)->admit(mgbfs::intake_graph(graph), visited, max_layers, slot);
+  })
+}
+
+extern "C" int mgbfs_batch_step(void* batch, uint32_t layers, uint32_t* active) {
+  if (!batch) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    const uint32_t a = static_cast<BatchEngine*>(batch)->step(layers);
+    if (active) *active = a;
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_run(void* batch) {
+  if (!batch) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    static_cast<BatchEngine*>(batch)->run_all();
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_poll(void* batch, int32_t* slots, uint32_t capacity, uint32_t* count) {
+  if (!batch || !count) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    const std::vector<int32_t> done = static_cast<BatchEngine*>(batch)->poll();
+    *count = static_cast<uint32_t>(done.size());
+    if (done.size() > capacity) return static_cast<int>(hipErrorInvalidValue);
+    if (slots)
+      for (size_t k = 0; k < done.size(); ++k) slots[k] = done[k];
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_result(void* batch, int32_t slot, uint64_t* sizes, uint32_t capacity, uint32_t* count,
+                                  uint32_t* status, uint32_t* error) {
+  if (!batch || !count) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    std::vector<uint64_t> s;
+    uint32_t st = 0, er = 0;
+    static_cast<BatchEngine*>(batch)->result(slot, s, st, er);
+    *count = static_cast<uint32_t>(s.size());
+    if (status) *status = st;
+    if (error) *error = er;
+    if (sizes)
+      for (uint32_t k = 0; k < capacity && k < s.size(); ++k) sizes[k] = s[k];
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_sample(void* batch, int32_t slot, uint64_t limit, int64_t* rows, uint64_t* count) {
+  if (!batch || !count) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    std::vector<int64_t> r;
+    uint64_t c = 0;
+    static_cast<BatchEngine*>(batch)->sample(slot, limit, r, c);
+    if (rows && !r.empty()) std::memcpy(rows, r.data(), r.size() * 8);
+    *count = c;
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_release(void* batch, int32_t slot) {
+  if (!batch) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    static_cast<BatchEngine*>(batch)->release(slot);
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_usage(void* batch, uint64_t* used, uint64_t* budget, uint32_t* slots) {
+  if (!batch) return hipErrorInvalidValue;
+  auto* b = static_cast<BatchEngine*>(batch);
+  if (used) *used = b->used();
+  if (budget) *budget = b->budget();
+  if (slots) *slots = b->slots();
+  return 0;
+}
+
+extern "C" void mgbfs_batch_destroy(void* batch) { delete static_cast<BatchEngine*>(batch); }

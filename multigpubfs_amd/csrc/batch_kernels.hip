@@ -1,0 +1,259 @@
+// Batched multi-instance BFS kernels.  One layer of every active instance
+// is three launches, all capturable in a hipGraph (no host readback, no
+// memset nodes):
+//   batch_tiles  : one block scans the per-instance tile counts
+//                  ceil(count / tile) of the active instances -> prefix.
+//   batch_expand : a persistent grid walks the global tile index space;
+//                  tile t belongs to the instance i with prefix[i] <= t <
+//                  prefix[i+1].  The instance descriptor (generator tables
+//                  included) is staged in LDS, children are claimed in the
+//                  instance's own visited set (Lehmer-rank bitmap or open
+//                  addressing hash with full-key equality) and new ones are
+//                  appended with one atomic per workgroup.
+//   batch_finish : one thread per instance records the layer size, swaps the
+//                  frontier buffers and retires finished instances.
+#include "batch.hpp"
+#include "kernels.hpp"
+
+namespace mgbfs {
+
+namespace {
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchItems = 2;  // parents per thread per tile (tile = 512 parents)
+
+__device__ __forceinline__ int64_t pymod64(int64_t x, int64_t m) {
+  int64_t r = x % m;
+  return r < 0 ? r + m : r;
+}
+
+__device__ __forceinline__ uint64_t batch_apply(const BatchInstance& s, uint64_t key, uint32_t g) {
+  if (s.action == kActPerm) return apply_packed(key, s.src[g], s.degree);
+  const uint64_t emask = (1ull << s.ebits) - 1;
+  uint64_t child = 0;
+  for (uint32_t i = 0; i < s.rows; ++i)
+    for (uint32_t j = 0; j < s.cols; ++j) {
+      uint64_t acc = 0;  // int64 arithmetic with two's-complement wrap
+      for (uint32_t k = 0; k < s.rows; ++k) {
+        const int64_t e = static_cast<int64_t>((key >> ((k * s.cols + j) * s.ebits)) & emask);
+        acc += static_cast<uint64_t>(static_cast<int64_t>(s.mat[g][i * s.rows + k])) * static_cast<uint64_t>(e);
+      }
+      const uint64_t v = static_cast<uint64_t>(pymod64(static_cast<int64_t>(acc), s.modulus));
+      child |= v << ((i * s.cols + j) * s.ebits);
+    }
+  return child;
+}
+
+// Returns true when this thread claimed `key` (first visit).
+__device__ __forceinline__ bool batch_claim(const BatchInstance& s, uint64_t key, uint32_t& err) {
+  if (s.visited == kVisBitmap) {
+    bool bad;
+    const uint64_t r = lehmer_rank_packed(key, s.degree, &bad);
+    if (bad || r >= s.bitmap_bits) {
+      err |= kErrDomain;
+      return false;
+    }
+    uint32_t* word = s.bitmap + (r >> 5);
+    const uint32_t bit = 1u << (r & 31u);
+    if (*word & bit) return false;
+    return !(__hip_atomic_fetch_or(word, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit);
+  }
+  uint64_t slot = mix64(key) & s.table_mask;
+  for (uint64_t probe = 0; probe <= s.table_mask; ++probe, slot = (slot + 1) & s.table_mask) {
+    const uint64_t seen = s.table[slot];
+    if (seen == key) return false;
+    if (seen != ~0ull) continue;
+    const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(s.table + slot), ~0ull,
+                                   static_cast<unsigned long long>(key));
+    if (old == ~0ull) return true;
+    if (old == key) return false;
+  }
+  err |= kErrTableFull;
+  return false;
+}
+
+__global__ void batch_reset_flag_kernel(uint32_t* flag) {
+  if (threadIdx.x == 0) *flag = 0;
+}
+
+__global__ __launch_bounds__(1024) void batch_tiles_kernel(const BatchInstance* inst, uint32_t slots, uint32_t tile,
+                                                           uint32_t* prefix) {
+  __shared__ uint32_t wave_sums[1024 / kWave];
+  const uint32_t i = threadIdx.x;
+  uint32_t v = 0;
+  if (i < slots && (inst[i].status & kInstActive) && !inst[i].error)
+    v = static_cast<uint32_t>((inst[i].count + tile - 1) / tile);
+  const uint32_t lane = i & (kWave - 1), wave = i / kWave;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off, kWave);
+    if (lane >= static_cast<uint32_t>(off)) incl += t;
+  }
+  if (lane == kWave - 1) wave_sums[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (uint32_t w = 0; w < blockDim.x / kWave; ++w) {
+    if (w < wave) before += wave_sums[w];
+    total += wave_sums[w];
+  }
+  if (i < slots) prefix[i] = before + incl - v;
+  if (i == 0) prefix[slots] = total;
+}
+
+__global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstance* inst, uint32_t slots,
+                                                                     const uint32_t* __restrict__ prefix,
+                                                                     uint32_t tile) {
+  __shared__ uint32_t spre[kBatchMaxSlots + 1];
+  __shared__ BatchInstance si;
+  __shared__ uint64_t parents[kBatchItems][kBatchThreads];
+  __shared__ uint32_t wave_sums[kBatchThreads / kWave];
+  __shared__ unsigned long long wg_base;
+  __shared__ int staged;
+  for (uint32_t k = threadIdx.x; k <= slots; k += kBatchThreads) spre[k] = prefix[k];
+  if (threadIdx.x == 0) staged = -1;
+  __syncthreads();
+  const uint32_t total = spre[slots];
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    // Instance owning tile t: the last i with spre[i] <= t.
+    uint32_t lo = 0, hi = slots - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (spre[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t i = lo;
+    if (static_cast<int>(i) != staged) {
+      __syncthreads();
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(inst + i);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&si);
+      for (uint32_t w = threadIdx.x; w < sizeof(BatchInstance) / 4; w += kBatchThreads) dst[w] = src[w];
+      if (threadIdx.x == 0) staged = static_cast<int>(i);
+      __syncthreads();
+    }
+    const uint64_t p0 = static_cast<uint64_t>(t - spre[i]) * tile;
+    const uint64_t p1 = p0 + tile < si.count ? p0 + tile : si.count;
+    const uint32_t G = si.generators;
+    uint32_t newmask = 0, err = 0;
+#pragma unroll
+    for (int it = 0; it < kBatchItems; ++it) {
+      const uint64_t p = p0 + static_cast<uint64_t>(it) * kBatchThreads + threadIdx.x;
+      const bool valid = p < p1;
+      const uint64_t key = valid ? si.front[p] : 0;
+      parents[it][threadIdx.x] = key;
+      if (!valid) continue;
+      for (uint32_t g = 0; g < G; ++g)
+        if (batch_claim(si, batch_apply(si, key, g), err)) newmask |= 1u << (it * kBatchMaxGens + g);
+    }
+    // Workgroup scan of new-child counts, one reservation per tile.
+    const uint32_t mine = __popc(newmask);
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    uint32_t incl = mine;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, kWave);
+      if (lane >= static_cast<uint32_t>(off)) incl += v;
+    }
+    if (lane == kWave - 1) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kBatchThreads / kWave; ++w) {
+      if (w < static_cast<int>(wave)) before += wave_sums[w];
+      sum += wave_sums[w];
+    }
+    if (threadIdx.x == 0) wg_base = sum ? atomicAdd(&inst[i].next_count, static_cast<unsigned long long>(sum)) : 0ull;
+    __syncthreads();
+    if (mine) {
+      uint64_t pos = wg_base + before + incl - mine;
+      if (pos + mine > si.cap) {
+        err |= kErrFutureCap;
+      } else {
+        uint32_t m = newmask;
+        while (m) {
+          const uint32_t b = __ffs(m) - 1;
+          m &= m - 1;
+          si.next[pos++] = batch_apply(si, parents[b / kBatchMaxGens][threadIdx.x], b % kBatchMaxGens);
+        }
+      }
+    }
+    if (err) atomicOr(&inst[i].error, err);
+    __syncthreads();  // wave_sums / wg_base / parents reuse
+  }
+}
+
+__global__ void batch_finish_kernel(BatchInstance* inst, uint32_t slots, uint32_t* any_active) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  BatchInstance& s = inst[i];
+  if (!(s.status & kInstActive)) return;
+  if (s.error) {
+    s.status = kInstDone;
+    return;
+  }
+  const uint64_t nc = s.next_count;
+  if (nc == 0) {
+    s.status = kInstDone;
+    return;
+  }
+  if (s.depth + 1 >= s.max_layers) {
+    s.error |= kErrCount;
+    s.status = kInstDone;
+    return;
+  }
+  s.layer_sizes[s.depth + 1] = nc;
+  uint64_t* f = s.front;
+  s.front = s.next;
+  s.next = f;
+  s.count = nc;
+  s.next_count = 0;
+  s.depth += 1;
+  atomicOr(any_active, 1u);
+}
+
+__global__ void batch_seed_kernel(BatchInstance* inst, uint32_t slot, uint64_t start_key) {
+  if (threadIdx.x != 0) return;
+  BatchInstance& s = inst[slot];
+  uint32_t err = 0;
+  if (!batch_claim(s, start_key, err)) err |= kErrDomain;
+  s.front[0] = start_key;
+  s.count = 1;
+  s.next_count = 0;
+  s.depth = 0;
+  s.layer_sizes[0] = 1;
+  s.error = err;
+  s.status = err ? kInstDone : kInstActive;
+}
+
+}  // namespace
+
+hipError_t batch_reset_flag(uint32_t* flag, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_reset_flag_kernel, dim3(1), dim3(64), 0, stream, flag);
+  return hipGetLastError();
+}
+
+hipError_t batch_tiles(BatchInstance* inst, uint32_t slots, uint32_t tile, uint32_t* prefix, hipStream_t stream) {
+  if (slots == 0 || slots > kBatchMaxSlots || tile == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_tiles_kernel, dim3(1), dim3(1024), 0, stream, inst, slots, tile, prefix);
+  return hipGetLastError();
+}
+
+hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t tile, uint32_t grid,
+                        uint32_t, hipStream_t stream) {
+  if (slots == 0 || slots > kBatchMaxSlots || tile != kBatchThreads * kBatchItems || grid == 0)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_expand_kernel, dim3(grid), dim3(kBatchThreads), 0, stream, inst, slots, prefix, tile);
+  return hipGetLastError();
+}
+
+hipError_t batch_finish(BatchInstance* inst, uint32_t slots, uint32_t* any_active, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_finish_kernel, dim3((slots + 255) / 256), dim3(256), 0, stream, inst, slots, any_active);
+  return hipGetLastError();
+}
+
+hipError_t batch_seed(BatchInstance* inst, uint32_t slot, uint64_t start_key, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_seed_kernel, dim3(1), dim3(64), 0, stream, inst, slot, start_key);
+  return hipGetLastError();
+}
+
+}  // namespace mgbfs

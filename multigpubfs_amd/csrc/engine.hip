@@ -413,6 +413,11 @@ static void put_error(char* text, size_t cap, const char* what) {
 
 static thread_local std::string g_last_error;
 
+namespace mgbfs {
+void set_last_error(const char* what) { g_last_error = what; }
+Graph intake_graph(const mgbfs_graph_desc* d) { return intake(d); }
+}  // namespace mgbfs
+
 #define MGBFS_GUARD(...)                                    \
   try {                                                     \
     __VA_ARGS__                                             \

@@ -1,0 +1,142 @@
+"""Generator-set sweep: many independent BFS instances on one GPU.
+
+`make_sweep` builds the 256-instance workload deterministically: instance i
+draws from a SplitMix64 stream seeded 0x4D47424600000000 + i; n is uniform
+in {9, 10}, k uniform in 2..5 generators, each a uniform random permutation
+(Fisher-Yates); even instances add every generator's inverse (undirected,
+inverse-closed), odd ones do not (directed, full visited history).
+
+`run_batched` advances all instances together through the batch engine
+(hipGraph-captured layer loop); `run_sequential` runs the same instances one
+at a time through the plain single-instance engine.  Both return per-instance
+layer tables, which must be identical.
+
+The reference has no batched kernel; its sweeps run instances one by one
+through resident workers (docs/reproduce-pair-sweeps.md:3,
+multigpubfs/generic_session.py:139-144).
+"""
+from __future__ import annotations
+
+import json
+import time
+
+from .graph_definition import GraphDefinition
+
+SWEEP_SEED = 0x4D47424600000000
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014) -- the public reference stream."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def below(self, bound: int) -> int:
+        return self.next() % bound
+
+
+def random_permutation(rng: SplitMix64, n: int) -> list[int]:
+    p = list(range(n))
+    for i in range(n - 1, 0, -1):
+        j = rng.below(i + 1)
+        p[i], p[j] = p[j], p[i]
+    return p
+
+
+def inverse(p: list[int]) -> list[int]:
+    q = [0] * len(p)
+    for i, v in enumerate(p):
+        q[v] = i
+    return q
+
+
+def make_instance(i: int) -> dict:
+    rng = SplitMix64(SWEEP_SEED + i)
+    n = 9 + rng.below(2)
+    k = 2 + rng.below(4)
+    gens = [random_permutation(rng, n) for _ in range(k)]
+    undirected = i % 2 == 0
+    if undirected:
+        gens = gens + [inverse(g) for g in gens]
+    g = GraphDefinition.permutation(gens, list(range(n)), name=f"sweep-{i}")
+    return g.to_dict()
+
+
+def make_sweep(count: int = 256) -> list[dict]:
+    return [make_instance(i) for i in range(count)]
+
+
+def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=16):
+    """All instances through one BatchEngine; returns (layer tables, seconds, info)."""
+    from ._native import BatchEngine
+    slots = max_slots or len(instances)
+    tables = [None] * len(instances)
+    with BatchEngine(device=device, max_slots=slots) as b:
+        t0 = time.perf_counter()
+        pending = list(range(len(instances)))
+        owner = {}
+        choices = {}
+        while pending or owner:
+            while pending:
+                slot = b.admit(instances[pending[0]], visited=visited)
+                if slot is None:
+                    break
+                owner[slot] = pending.pop(0)
+            b.step(layers)
+            for slot in b.poll():
+                sizes, status, error = b.result(slot)
+                if error:
+                    raise RuntimeError(f"BATCH_INSTANCE_ERROR {owner[slot]}: {error}")
+                tables[owner.pop(slot)] = sizes
+                b.release(slot)
+        seconds = time.perf_counter() - t0
+        for idx in range(min(len(instances), 4)):
+            choices[idx] = b.plan(instances[idx], visited=visited)[1]
+    return tables, seconds, {"visited_sample": choices}
+
+
+def run_sequential(instances, *, device=0, backend="permutation_bitmap"):
+    """Plain engine, one instance at a time (create, run to completion,
+    destroy).  The Lehmer-bitmap backend is the plain engine's fastest exact
+    choice for these S9/S10 instances, directed or not."""
+    from ._native import NativeEngine
+    tables = []
+    t0 = time.perf_counter()
+    for g in instances:
+        gd = GraphDefinition(g)
+        with NativeEngine(gd, device=device, backend=backend) as e:
+            status = e.run()
+            if status != 1:
+                raise RuntimeError(f"SEQUENTIAL_INSTANCE_STATUS {status}")
+            tables.append(e.layer_sizes())
+    return tables, time.perf_counter() - t0
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="batched vs one-at-a-time generator-set sweep")
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--write-instances", help="write the instance list as JSON and exit")
+    args = ap.parse_args(argv)
+    inst = make_sweep(args.count)
+    if args.write_instances:
+        with open(args.write_instances, "w") as f:
+            json.dump(inst, f)
+        return 0
+    batched, tb, info = run_batched(inst)
+    seq, ts = run_sequential(inst)
+    print(json.dumps({"instances": len(inst), "batched_s": tb, "sequential_s": ts, "speedup": ts / tb,
+                      "identical": batched == seq, **info}))
+    return 0 if batched == seq else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -251,3 +251,33 @@ def test_cli_key_info():
     proc = cli("key-info")
     doc = json.loads(proc.stdout)
     assert proc.returncode == 0 and "graph" in doc["commands"] and doc["device_arch"] == "gfx950"
+
+
+# ------------------------------------------------------ sweep / serving inputs
+
+def test_sweep_instances_are_deterministic_and_shaped():
+    from multigpubfs_amd.sweep import make_sweep
+    a, b = make_sweep(32), make_sweep(32)
+    assert a == b
+    for i, g in enumerate(a):
+        gd = GraphDefinition(g)
+        n, k = gd.state_elements, gd.generator_count
+        assert n in (9, 10)
+        assert (2 <= k <= 5) if i % 2 else (4 <= k <= 10 and gd.inverse_closed())
+
+
+def test_serving_job_stream_mix_and_rate():
+    from multigpubfs_amd.serving import make_jobs
+    jobs = make_jobs(2000)
+    assert jobs == make_jobs(2000)
+    perm = [j for j in jobs if j["graph"]["action"]["kind"] == "permutation"]
+    assert 0.65 < len(perm) / len(jobs) < 0.75
+    assert 40 * 0.9 < len(jobs) / jobs[-1]["arrival"] < 40 * 1.1
+    for j in jobs:
+        gd = GraphDefinition(j["graph"])
+        assert gd.inverse_closed()
+        if gd.action["kind"] == "permutation":
+            assert 8 <= gd.state_elements <= 12 and 2 <= gd.generator_count <= 4
+        else:
+            p = gd.action["generators"][0]["modulo"]
+            assert 101 <= p <= 521 and all(p % d for d in range(2, p))

@@ -1,0 +1,95 @@
+"""Batched multi-instance engine: every instance's layer table must equal the
+oracle's (committed golden tables) and the one-at-a-time engine's."""
+import json
+import math
+from pathlib import Path
+
+import pytest
+
+from multigpubfs_amd import GraphDefinition, NativeEngine
+from multigpubfs_amd._native import BatchEngine
+from multigpubfs_amd.sweep import make_sweep, run_batched, run_sequential
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "sweep256.json").read_text())["instances"]
+
+
+def test_golden_fixture_matches_generator():
+    assert [x["graph"] for x in GOLDEN] == make_sweep(256)
+
+
+def test_full_sweep_batched_matches_golden_tables():
+    tables, seconds, _ = run_batched([x["graph"] for x in GOLDEN])
+    assert tables == [x["layer_sizes"] for x in GOLDEN]
+
+
+@pytest.mark.parametrize("visited", ["bitmap", "hash"])
+def test_forced_visited_backends_agree(visited):
+    subset = GOLDEN[:24]
+    tables, _, _ = run_batched([x["graph"] for x in subset], visited=visited)
+    assert tables == [x["layer_sizes"] for x in subset]
+
+
+def test_sequential_plain_engine_matches_batched():
+    subset = GOLDEN[:16]
+    seq, _ = run_sequential([x["graph"] for x in subset])
+    assert seq == [x["layer_sizes"] for x in subset]
+
+
+def sl2(p):
+    gens = [([1, 1, 0, 1], p), ([1, 0, 1, 1], p), ([1, p - 1, 0, 1], p), ([1, 0, p - 1, 1], p)]
+    return GraphDefinition.matrix(2, 2, gens, [1, 0, 0, 1], expected_max_unique_states=p * (p * p - 1))
+
+
+def test_matrix_instances_in_batch_match_oracle():
+    graphs = [sl2(p) for p in (5, 7, 11, 13, 101)]
+    tables, _, _ = run_batched([g.to_dict() for g in graphs])
+    for g, t in zip(graphs, tables):
+        p = g.action["generators"][0]["modulo"]
+        assert sum(t) == p * (p * p - 1)
+        if p <= 13:
+            assert t == [len(x) for x in O.c_exact_layers(g.to_dict(), p * (p * p - 1))]
+
+
+def test_continuous_admission_results_equal_alone():
+    # Jobs admitted while others are mid-BFS get the same tables as alone.
+    jobs = [x["graph"] for x in GOLDEN[:12]] + [sl2(7).to_dict(), sl2(11).to_dict()]
+    alone = [run_batched([j])[0][0] for j in jobs]
+    with BatchEngine(max_slots=8) as b:
+        owner, done, queue = {}, {}, list(range(len(jobs)))
+        while queue or owner:
+            if queue:  # admit one job per step so arrivals interleave with running ones
+                slot = b.admit(jobs[queue[0]])
+                if slot is not None:
+                    owner[slot] = queue.pop(0)
+            b.step(2)
+            for slot in b.poll():
+                sizes, status, error = b.result(slot)
+                assert error == 0
+                done[owner.pop(slot)] = sizes
+                b.release(slot)
+    assert [done[i] for i in range(len(jobs))] == alone
+
+
+def test_budget_admission_refuses_when_full():
+    g = GOLDEN[1]["graph"]  # n = 9 or 10
+    with BatchEngine(max_slots=4, vram_budget=1) as b:
+        assert b.admit(g) is None
+    with BatchEngine(max_slots=2) as b:
+        assert b.admit(g) is not None and b.admit(g) is not None
+        assert b.admit(g) is None  # no free slot
+        nbytes, visited = b.plan(g)
+        assert nbytes > 0 and visited in ("bitmap", "hash")
+
+
+def test_batch_samples_are_last_layer_states():
+    g = GraphDefinition.permutation(O.lrx_generators(8), list(range(8)))
+    ref = O.packed_bfs(8, keep_keys=True)
+    with BatchEngine(max_slots=2) as b:
+        slot = b.admit(g.to_dict())
+        b.run()
+        sizes, status, error = b.result(slot)
+        rows = b.sample(slot, 8)
+    assert sizes == ref["layers"] and error == 0
+    assert sorted(O.pack(r) for r in rows) == sorted(ref["keys"][-1])
