@@ -41,6 +41,37 @@ def cpu_baseline(n: int, seconds_target: float) -> dict:
                       f"in {dt:.2f} s (oracle/mgbfs_oracle.c orc_packed_bfs, {cores} threads)"}
 
 
+def sweep_section() -> dict:
+    """256 S9/S10 instances: batched (one hipGraph layer loop) vs one at a time."""
+    from multigpubfs_amd.sweep import run_batched, run_sequential
+    golden = json.loads((ROOT / "tests" / "golden" / "sweep256.json").read_text())["instances"]
+    inst = [x["graph"] for x in golden]
+    run_batched(inst[:8])  # warm the kernels and the allocator
+    batched, tb, info = run_batched(inst)
+    seq, ts = run_sequential(inst)
+    want = [x["layer_sizes"] for x in golden]
+    return {"instances": len(inst), "batched_s": tb, "sequential_s": ts,
+            "batched_bfs_per_s": len(inst) / tb, "sequential_bfs_per_s": len(inst) / ts,
+            "speedup": ts / tb, "identical_tables": batched == seq, "golden_exact": batched == want,
+            "states": sum(sum(w) for w in want)}
+
+
+def serving_section(count: int) -> dict:
+    """Poisson stream (40 jobs/s) through the continuous-batching loop."""
+    from multigpubfs_amd.serving import make_jobs, serve
+    jobs = make_jobs(count)
+    report = serve(jobs, vram_budget=200 * 10 ** 9)
+    sl2_exact = True
+    for job in jobs:
+        a = job["graph"]["action"]
+        if a["kind"] == "matrix":
+            p = a["generators"][0]["modulo"]
+            sl2_exact &= sum(report["results"][job["id"]]["layer_sizes"]) == p * (p * p - 1)
+    out = {k: v for k, v in report.items() if k != "results"}
+    out.update(vram_budget_bytes=200 * 10 ** 9, arrival_rate_per_s=40, sl2_orders_exact=sl2_exact)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +81,9 @@ def main(argv=None):
     ap.add_argument("--backend", default="sorted")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 256-instance batched sweep")
+    ap.add_argument("--no-serve", action="store_true", help="skip the 2,000-job serving run")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -155,10 +189,17 @@ def main(argv=None):
                      "phases": phases},
     }
     engine.close()
+    if not args.no_sweep:
+        result["sweep"] = sweep_section()
+        print(f"[bench] sweep {result['sweep']}", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        result["serving"] = serving_section(args.serve_jobs)
+        print(f"[bench] serving {result['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
-    return 0 if exact else 1
+    ok = exact and result.get("sweep", {}).get("golden_exact", True) and not result.get("serving", {}).get("errors")
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

@@ -211,6 +211,9 @@ class NativeEngine:
         return list(buf[:n.value])
 
     def sample(self, which: int = 0, limit: int = 1000) -> list[list[int]]:
+        sizes = self.layer_sizes()
+        available = sizes[-1] if which == 0 else (sizes[-2] if len(sizes) > 1 else 0)
+        limit = min(limit, available)
         rows = (C.c_int64 * max(1, limit * self.elements))()
         n = C.c_uint64()
         rc = self._lib.mgbfs_engine_sample(self._h, which, limit, rows, C.byref(n))
@@ -359,6 +362,8 @@ class BatchEngine:
         return list(buf[:n.value]), status.value, error.value
 
     def sample(self, slot, width, limit=1000):
+        sizes, _, _ = self.result(slot)
+        limit = min(limit, sizes[-1] if sizes else 0)
         rows = (C.c_int64 * max(1, limit * width))()
         n = C.c_uint64()
         self._check(self._lib.mgbfs_batch_sample(self._h, slot, limit, rows, C.byref(n)), "BATCH_SAMPLE")
