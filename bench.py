@@ -46,14 +46,19 @@ def sweep_section() -> dict:
     from multigpubfs_amd.sweep import run_batched, run_sequential
     golden = json.loads((ROOT / "tests" / "golden" / "sweep256.json").read_text())["instances"]
     inst = [x["graph"] for x in golden]
-    run_batched(inst[:8])  # warm the kernels and the allocator
-    batched, tb, info = run_batched(inst)
+    from multigpubfs_amd.autotune import choose_profile
+    t0 = time.perf_counter()
+    profile = choose_profile(inst, use_cache=False)
+    tune_s = time.perf_counter() - t0
+    run_batched(inst[:8], profile=profile)  # warm the kernels and the allocator
+    batched, tb, info = run_batched(inst, profile=profile)
     seq, ts = run_sequential(inst)
     want = [x["layer_sizes"] for x in golden]
     return {"instances": len(inst), "batched_s": tb, "sequential_s": ts,
             "batched_bfs_per_s": len(inst) / tb, "sequential_bfs_per_s": len(inst) / ts,
             "speedup": ts / tb, "identical_tables": batched == seq, "golden_exact": batched == want,
-            "states": sum(sum(w) for w in want)}
+            "states": sum(sum(w) for w in want), "autotune_seconds": tune_s,
+            "profile": {k: profile[k] for k in ("grid_per_cu", "layers_per_graph", "visited")}}
 
 
 def serving_section(count: int) -> dict:

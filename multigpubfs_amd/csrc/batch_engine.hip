@@ -8,6 +8,7 @@
 #include <map>
 #include <memory>
 #include <vector>
+#include "arena.hpp"
 #include "batch.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
@@ -26,42 +27,78 @@ uint64_t next_pow2(uint64_t x) {
   return p;
 }
 
-// Reuses device allocations across admitted jobs (exact byte size match).
-class BufferPool {
+// Device memory for admitted jobs: a few large chunks, sub-allocated by
+// best-fit ranges, so admission costs no hipMalloc once the arena has grown
+// to the working set.  Chunks grow geometrically (kArenaFirst doubling to
+// kArenaMax) within the engine's budget; a chunk that is entirely free is
+// handed back to the device only when the arena holds more than the budget.
+constexpr uint64_t kArenaFirst = 1ull << 30;
+constexpr uint64_t kArenaMax = 32ull << 30;
+
+struct Block {
+  char* ptr = nullptr;
+  uint32_t chunk = 0;
+  uint64_t offset = 0, bytes = 0;
+};
+
+class DeviceArena {
  public:
-  DeviceBuffer take(uint64_t bytes) {
-    auto it = free_.find(bytes);
-    if (it != free_.end()) {
-      DeviceBuffer b = std::move(it->second);
-      free_.erase(it);
-      pooled_ -= bytes;
-      return b;
+  Block take(uint64_t bytes, uint64_t budget) {
+    bytes = arena_round(bytes);
+    for (uint32_t c = 0; c < chunks_.size(); ++c) {
+      uint64_t off = 0;
+      if (chunks_[c].buf.bytes() && chunks_[c].ranges.take(bytes, &off))
+        return Block{chunks_[c].buf.as<char>() + off, c, off, bytes};
     }
-    DeviceBuffer b;
-    b.allocate(bytes);
-    return b;
+    uint64_t size = std::max(bytes, std::min(next_, kArenaMax));
+    if (reserved_ + size > budget && reserved_ + bytes <= budget) size = budget - reserved_;
+    size = std::max(bytes, size / kArenaAlign * kArenaAlign);
+    uint32_t c = 0;
+    while (c < chunks_.size() && chunks_[c].buf.bytes()) ++c;
+    if (c == chunks_.size()) chunks_.emplace_back();
+    chunks_[c].buf.allocate(size);
+    chunks_[c].ranges = RangeSet(size);
+    reserved_ += size;
+    next_ = std::min(2 * next_, kArenaMax);
+    uint64_t off = 0;
+    chunks_[c].ranges.take(bytes, &off);
+    return Block{chunks_[c].buf.as<char>() + off, c, off, bytes};
   }
-  void give(DeviceBuffer&& b) {
-    if (!b.bytes()) return;
-    pooled_ += b.bytes();
-    free_.emplace(b.bytes(), std::move(b));
+  void give(Block& b, uint64_t budget) {
+    if (!b.ptr) return;
+    Chunk& ch = chunks_[b.chunk];
+    ch.ranges.give(b.offset, b.bytes);
+    if (ch.ranges.empty() && reserved_ > budget) {
+      reserved_ -= ch.buf.bytes();
+      ch.buf.release();
+      ch.ranges = RangeSet();
+    }
+    b = Block{};
   }
   void trim() {
-    free_.clear();
-    pooled_ = 0;
+    for (Chunk& ch : chunks_)
+      if (ch.buf.bytes() && ch.ranges.empty()) {
+        reserved_ -= ch.buf.bytes();
+        ch.buf.release();
+        ch.ranges = RangeSet();
+      }
   }
-  uint64_t pooled() const { return pooled_; }
+  uint64_t reserved() const { return reserved_; }
 
  private:
-  std::multimap<uint64_t, DeviceBuffer> free_;
-  uint64_t pooled_ = 0;
+  struct Chunk {
+    DeviceBuffer buf;
+    RangeSet ranges;
+  };
+  std::vector<Chunk> chunks_;
+  uint64_t reserved_ = 0, next_ = kArenaFirst;
 };
 
 struct Slot {
   bool used = false;
   bool reported = false;
   uint64_t planned = 0;
-  DeviceBuffer front, next, sizes, visited;
+  Block block;  // front, next, layer sizes, visited set (256-byte aligned parts)
 };
 
 }  // namespace
@@ -104,8 +141,8 @@ class BatchEngine {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (host_flag_) (void)hipHostFree(host_flag_);
-    for (auto& s : slots_) s = Slot{};
-    pool_.trim();
+    slots_.clear();
+    arena_.trim();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -156,26 +193,26 @@ class BatchEngine {
     if (slot < 0) return MGBFS_BATCH_FULL;
     MGBFS_HIP(hipSetDevice(device_));
     Slot& sl = slots_[slot];
-    sl.front = pool_.take(p.cap * 8);
-    sl.next = pool_.take(p.cap * 8);
-    sl.sizes = pool_.take(8ull * p.max_layers);
-    sl.visited = pool_.take(p.visited_bytes);
+    const uint64_t keys_bytes = arena_round(p.cap * 8), sizes_bytes = arena_round(8ull * p.max_layers);
+    sl.block = arena_.take(2 * keys_bytes + sizes_bytes + p.visited_bytes, budget_);
+    char* base = sl.block.ptr;
     BatchInstance& d = host_inst_[slot];
     std::memset(&d, 0, sizeof d);
-    d.front = sl.front.as<uint64_t>();
-    d.next = sl.next.as<uint64_t>();
+    d.front = reinterpret_cast<uint64_t*>(base);
+    d.next = reinterpret_cast<uint64_t*>(base + keys_bytes);
     d.cap = p.cap;
-    d.layer_sizes = sl.sizes.as<uint64_t>();
+    d.layer_sizes = reinterpret_cast<uint64_t*>(base + 2 * keys_bytes);
+    char* visited_base = base + 2 * keys_bytes + sizes_bytes;
     d.max_layers = p.max_layers;
     d.visited = p.visited;
     d.generators = g.generators;
     uint64_t start_key = 0;
     if (p.visited == kVisBitmap) {
-      d.bitmap = sl.visited.as<uint32_t>();
+      d.bitmap = reinterpret_cast<uint32_t*>(visited_base);
       d.bitmap_bits = factorial_u64(g.elements);
       MGBFS_HIP(fill_u32(d.bitmap, 0u, p.visited_bytes / 4, stream_));
     } else {
-      d.table = sl.visited.as<uint64_t>();
+      d.table = reinterpret_cast<uint64_t*>(visited_base);
       d.table_mask = p.visited_bytes / 8 - 1;
       MGBFS_HIP(fill_u64(d.table, ~0ull, p.visited_bytes / 8, stream_));
     }
@@ -198,10 +235,11 @@ class BatchEngine {
           d.mat[k][e] = static_cast<int32_t>(g.matrices[static_cast<size_t>(k) * g.rows * g.rows + e]);
       for (uint32_t e = 0; e < g.elements; ++e) start_key |= static_cast<uint64_t>(g.start[e]) << (e * eb);
     }
-    // The stream is idle between steps: descriptor upload, then seed.
+    // Stream order makes the upload, fill and seed land before the next
+    // step's graph; host_inst_[slot] stays untouched until the next refresh,
+    // which synchronises, so no wait is needed here.
     MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
     MGBFS_HIP(batch_seed(inst_.as<BatchInstance>(), static_cast<uint32_t>(slot), start_key, stream_));
-    MGBFS_HIP(hipStreamSynchronize(stream_));
     sl.used = true;
     sl.reported = false;
     sl.planned = p.bytes;
@@ -273,16 +311,12 @@ class BatchEngine {
     Slot& sl = slots_[slot];
     BatchInstance& d = host_inst_[slot];
     std::memset(&d, 0, sizeof d);
+    // The cleared descriptor is uploaded before any later use of the block
+    // (same stream), so the range can be reused at once.
     MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
-    MGBFS_HIP(hipStreamSynchronize(stream_));
-    pool_.give(std::move(sl.front));
-    pool_.give(std::move(sl.next));
-    pool_.give(std::move(sl.sizes));
-    pool_.give(std::move(sl.visited));
+    arena_.give(sl.block, budget_);
     used_ -= sl.planned;
     sl = Slot{};
-    // Pooled buffers count against the budget only while they fit beside live jobs.
-    if (used_ + pool_.pooled() > budget_) pool_.trim();
   }
 
   uint64_t used() const { return used_; }
@@ -323,7 +357,7 @@ class BatchEngine {
   std::vector<BatchInstance> host_inst_;
   std::vector<Slot> slots_;
   std::map<int32_t, uint32_t> kinds_, rows_, cols_;
-  BufferPool pool_;
+  DeviceArena arena_;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
 };

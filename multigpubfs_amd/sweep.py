@@ -74,12 +74,18 @@ def make_sweep(count: int = 256) -> list[dict]:
     return [make_instance(i) for i in range(count)]
 
 
-def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=16):
-    """All instances through one BatchEngine; returns (layer tables, seconds, info)."""
-    from ._native import BatchEngine
+def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=16, profile=None):
+    """All instances through one BatchEngine; returns (layer tables, seconds, info).
+    `profile` (from autotune.choose_profile) sets the launch shape."""
+    from ._native import BatchEngine, device_info
     slots = max_slots or len(instances)
     tables = [None] * len(instances)
-    with BatchEngine(device=device, max_slots=slots) as b:
+    grid = layers_per_graph = 0
+    if profile:
+        grid = profile["grid_per_cu"] * device_info(device)["compute_units"]
+        layers_per_graph = profile["layers_per_graph"]
+        visited = profile["visited"]
+    with BatchEngine(device=device, max_slots=slots, grid=grid, layers_per_graph=layers_per_graph) as b:
         t0 = time.perf_counter()
         pending = list(range(len(instances)))
         owner = {}

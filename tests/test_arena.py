@@ -1,0 +1,19 @@
+"""Host-only check of the batch engine's device-arena range bookkeeping
+(multigpubfs_amd/csrc/arena.hpp), compiled with g++ and run on the CPU."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
+def test_range_set_matches_occupancy_model(tmp_path):
+    exe = tmp_path / "arena_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", str(ROOT / "multigpubfs_amd" / "csrc"),
+                    str(ROOT / "tests" / "native" / "arena_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "arena ok" in out.stdout

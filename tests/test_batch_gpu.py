@@ -93,3 +93,17 @@ def test_batch_samples_are_last_layer_states():
         rows = b.sample(slot, 8)
     assert sizes == ref["layers"] and error == 0
     assert sorted(O.pack(r) for r in rows) == sorted(ref["keys"][-1])
+
+
+def test_autotuned_profile_keeps_tables_exact(tmp_path, monkeypatch):
+    from multigpubfs_amd.autotune import CANDIDATES, DEFAULT, choose_profile
+    monkeypatch.setenv("MGBFS_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    subset = GOLDEN[:32]
+    inst = [x["graph"] for x in subset]
+    profile = choose_profile(inst, pilot=8)
+    chosen = {k: profile[k] for k in DEFAULT}
+    assert chosen == DEFAULT or chosen in CANDIDATES
+    assert all(t.get("identical", True) for t in profile["trials"])
+    tables, _, _ = run_batched(inst, profile=profile)
+    assert tables == [x["layer_sizes"] for x in subset]
+    assert choose_profile(inst, pilot=8)["cached"] is True
