@@ -23,6 +23,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE per kernel for one full BFS,
+# committed beside the text summaries (profiles/*_s13_lrx.md).
+TRAFFIC_FILES = {"sorted": ROOT / "profiles" / "sorted_engine_s13_lrx.json",
+                 "bitmap": ROOT / "profiles" / "bitmap_expand_kernel_s13_lrx.json"}
 REFERENCE_EDGES_PER_S = 18681062400 / 11.716894  # 8x H200 median, docs/validation/lrx13-eight-h200.md:26
 
 
@@ -152,9 +156,8 @@ def main(argv=None):
         names = ["bitmap_expand_kernel"]
         models = ["8F frontier read + 8N next-layer write (bitmap probes excluded)"]
     else:
-        names = ["expand_count_kernel+expand_scatter_kernel", "partition_count_kernel+partition_scatter_kernel",
-                 "unique_diff_kernel", "compact_kernel"]
-        models = ["16F + 8C", "24C", "8C + 8F + 8F_prev + 8N", "16N"]
+        names = ["expand_scatter_kernel", "partition_scatter_kernel", "unique_diff_kernel", "compact_kernel"]
+        models = ["8F + 8C", "16C", "8C + 8F + 8F_prev + 8N", "16N"]
     phases = []
     for i, name in enumerate(names):
         if phase_s[i] <= 0:
@@ -163,6 +166,14 @@ def main(argv=None):
         phases.append({"kernel": name, "ms_per_step": 1e3 * phase_s[i] / args.steps,
                        "bytes_per_step": phase_b[i] // args.steps, "achieved_gbs": gbs, "bytes_model": models[i]})
     dom = max(phases, key=lambda p: p["ms_per_step"])
+    traffic, traffic_source = None, None
+    tf = TRAFFIC_FILES.get("bitmap" if info["backend"] == 1 else "sorted")
+    if n == 13 and tf is not None and tf.exists():
+        prof = json.loads(tf.read_text())
+        k = prof["kernels"].get(dom["kernel"])
+        if k:
+            traffic = (k["fetch_bytes"] + k["write_bytes"]) / k["dispatches"]
+            traffic_source = f"{tf.relative_to(ROOT)} ({prof['library']})"
     per_launch_layers = layers * args.steps
     result = {
         "metric": f"expanded edges/s on S{n} LRX BFS",
@@ -186,7 +197,7 @@ def main(argv=None):
                    "baseline": "reference 8x H200, 11.716894 s (docs/validation/lrx13-eight-h200.md:26)"},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["achieved_gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": traffic, "traffic_source": traffic_source,
                      "avg_launch_ms": dom["ms_per_step"] / layers,
                      "bytes_per_launch": dom["bytes_per_step"] / layers,
                      "bytes_model": dom["bytes_model"],

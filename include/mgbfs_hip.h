@@ -53,6 +53,8 @@ extern "C" {
                                              so large layers take the overflow-retry path */
 #define MGBFS_FLAG_TEST_COLLIDE_HASH 2u   /* generic backend: hash bits 0, every state
                                              collides and full-state equality decides */
+#define MGBFS_FLAG_TEST_TIGHT_SLABS 4u    /* sorted backend: owner-lane slabs start with
+                                             no slack, so lane overflows take the retry path */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are

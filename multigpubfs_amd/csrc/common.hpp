@@ -17,6 +17,7 @@ constexpr uint32_t kErrBadRef = 8u;         // invalid row reference
 constexpr uint32_t kErrCount = 32u;         // received count above bound
 constexpr uint32_t kErrBucketCap = 64u;     // sorted engine: bucket above LDS capacity
 constexpr uint32_t kErrDomain = 128u;       // state outside the packed/rank domain
+constexpr uint32_t kErrLaneSlab = 256u;     // sorted engine: owner-lane slab full (retried with more room)
 
 // Generator table passed by value: wave-uniform indices become scalar loads.
 //
@@ -162,6 +163,12 @@ __host__ __device__ __forceinline__ uint64_t unmix64(uint64_t x) {
 // Multiplicative (Fibonacci-style) hashing mod 2^B puts every key bit into
 // the top bits (the bucket ids); the final xorshift folds the top half into
 // the low bits used as LDS slot hashes.  Both steps are bijections on B bits.
+// Bijection on `bits`-bit keys: multiply by an odd constant mod 2^bits,
+// then xorshift by shift >= bits/2 (its own inverse).  The top bits (bucket
+// ids) are those of the product (Fibonacci hashing), which spreads whole
+// layers evenly; callers must not rely on keys that differ only in a few
+// low nibbles landing far apart (the sorted backend samples its routing
+// tiles across the whole frontier for that reason).
 __host__ __device__ __forceinline__ uint64_t keymix(uint64_t x, const KeyMix& k) {
   x = (x * k.m1) & k.mask;
   x ^= x >> k.shift;

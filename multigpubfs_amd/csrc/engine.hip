@@ -382,8 +382,9 @@ uint64_t state_space_bound(const Graph& g) {
 static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
   uint32_t backend = o.backend;
   if (backend == MGBFS_BACKEND_AUTO) {
-    if (g.packable() && g.inverse_closed) backend = MGBFS_BACKEND_SORTED;
-    else if (g.packable() && g.distinct_perm()) backend = MGBFS_BACKEND_BITMAP;
+    // The sorted backend keeps at least 4 tag bits beside the 4n key bits.
+    if (g.packable() && g.inverse_closed && g.elements <= 15) backend = MGBFS_BACKEND_SORTED;
+    else if (g.packable() && g.distinct_perm() && g.elements <= 15) backend = MGBFS_BACKEND_BITMAP;
     else backend = MGBFS_BACKEND_GENERIC;
   }
   switch (backend) {
@@ -391,7 +392,7 @@ static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
       if (!g.packable() || !g.distinct_perm()) throw std::runtime_error("BITMAP_UNSUPPORTED_GRAPH");
       return make_bitmap_engine(g, o);
     case MGBFS_BACKEND_SORTED:
-      if (!g.packable()) throw std::runtime_error("SORTED_UNSUPPORTED_GRAPH");
+      if (!g.packable() || g.elements > 15) throw std::runtime_error("SORTED_UNSUPPORTED_GRAPH");
       return make_sorted_engine(g, o);
     case MGBFS_BACKEND_GENERIC:
       return make_generic_engine(g, o, state_space_bound(g));

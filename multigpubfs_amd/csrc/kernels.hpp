@@ -17,6 +17,7 @@ constexpr uint32_t kMaxBucketBits = 20;   // sub-buckets per layer: 2^20
 constexpr uint32_t kMaxFineBits = 10;     // bank partition bits beyond the layer's K
 constexpr uint32_t kUniqueSlots = 8192;   // LDS hash slots per sub-bucket
 constexpr uint32_t kUniqueCap = 4096;     // candidates per sub-bucket (load factor <= 0.5)
+constexpr uint32_t kSlabStride = 4096;    // entries between sub-bucket slab starts (>= kUniqueCap)
 
 // Stored-value layout of the sorted backend: (keymix(key) << tagbits) | tag.
 struct SortedLayout {
@@ -31,19 +32,26 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out, uin
                               hipStream_t stream);
 hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
                        hipStream_t stream);
+// Children of bank[0..F) routed by their top K1 bits into slabs: slab
+// (lane * P + blockIdx % P) of `cap` entries, `stride` entries apart,
+// cursors[] counts per slab
+// (zeroed by the caller); a run beyond `cap` is dropped and sets err_bit.
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan,
-                               uint32_t* scan_ws, uint64_t* out, hipStream_t stream);
-hipError_t sorted_partition(const uint64_t* in, const uint32_t* scan1, uint32_t NB1, uint32_t K1, uint32_t K2,
-                            uint32_t NC, uint32_t* counts, uint32_t* scan2, uint32_t* scan_ws, uint64_t* out,
-                            hipStream_t stream);
-hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, const uint64_t* cur,
+                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
+                               hipStream_t stream);
+// Lane slabs (lcount per slab, capacity and stride lcap) -> sub-bucket
+// slabs of kUniqueCap entries, kSlabStride apart (scount per sub-bucket,
+// zeroed by the caller).
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
+                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, hipStream_t stream);
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
                               const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, uint32_t tagbits, uint32_t grid_limit,
                               hipStream_t stream);
-hipError_t sorted_compact(const uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, uint32_t Ko,
-                          const uint32_t* off, uint64_t* bank, hipStream_t stream);
+hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, uint64_t* bank,
+                          hipStream_t stream);
 
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);
 hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);

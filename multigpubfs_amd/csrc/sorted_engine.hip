@@ -1,8 +1,9 @@
 // Host runtime of the sort-unique backend with three rotating VRAM banks.
 //
-// Layer loop (one host readback per layer, plus a retry when a sub-bucket
-// exceeds the LDS capacity): expand+route -> partition -> unique_diff ->
-// scan of fine counts -> read N -> compact into the next bank -> rotate.
+// Layer loop (one host readback per layer, plus a retry when a slab
+// overflows): expand+route into owner-lane slabs -> partition into
+// sub-bucket slabs -> unique_diff -> scan of fine counts -> read N ->
+// compact into the next bank -> rotate.
 // Retention is exact for inverse-closed generator sets: every neighbour of
 // layer d lies in layer d-1, d or d+1 (graph_definition.py:84-92; retention
 // rule history_layers = 3, crates/mgbfs-runtime/src/generic_distributed_run.rs:40).
@@ -24,6 +25,18 @@ uint32_t ceil_log2(uint64_t x) {
 
 constexpr uint64_t kTargetBucket = 3072;  // average candidates per sub-bucket (cap 4096)
 constexpr uint32_t kPhaseCount = 4;       // expand, partition, unique, compact
+constexpr uint32_t kLaneParts = 32;       // slabs per owner lane (cursor contention / P)
+
+// Slab capacity for `mean` expected entries: mean + slack * (mean / 4 +
+// 16 sqrt(mean) + 1024), rounded to 64 entries.  A lane part's children come
+// from contiguous tiles of a frontier ordered by mixed-key bits, and the
+// mixer's top bits are linear in the key, so lane counts spread wider than
+// a binomial; slack doubles after an overflow.
+uint32_t slab_capacity(double mean, uint32_t slack) {
+  const double c = mean + slack * (mean / 4.0 + 16.0 * std::sqrt(mean) + 1024.0);
+  const uint64_t r = (static_cast<uint64_t>(c) + 63) / 64 * 64;
+  return static_cast<uint32_t>(std::min<uint64_t>(r, 0xffffffc0ull));
+}
 
 }  // namespace
 
@@ -83,6 +96,7 @@ class SortedEngine final : public Engine {
     MGBFS_HIP(hipSetDevice(device_));
     cur_ = 0;
     prev_ = -1;
+    slack_ = (flags_ & MGBFS_FLAG_TEST_TIGHT_SLABS) ? 0 : 1;
     for (int b = 0; b < 3; ++b) count_[b] = 0;
     count_[0] = 1;
     ko_[0] = kMaxBucketBits;
@@ -113,15 +127,11 @@ class SortedEngine final : public Engine {
       terminal = true;
       return MGBFS_RESOURCE;
     }
-    if (!ensure_candidates(Cmax)) {
-      *fatal = kErrFutureCap;
-      terminal = true;
-      return MGBFS_RESOURCE;
-    }
     const int nxt = next_bank();
-    // With parent-edge tags every tagged parent skips exactly one generator.
-    const uint64_t Cest = (layout_.prune && layers.size() > 1) ? F * (G - 1 ? G - 1 : 1) : Cmax;
-    uint32_t K = Cest <= kTargetBucket ? 0 : ceil_log2((Cest + kTargetBucket - 1) / kTargetBucket);
+    // Every parent yields exactly G children, or G-1 when it carries a tag.
+    const bool all_tagged = layout_.prune && layers.size() > 1;
+    const uint64_t C = all_tagged ? F * (G - 1) : Cmax;
+    uint32_t K = C <= kTargetBucket ? 0 : ceil_log2((C + kTargetBucket - 1) / kTargetBucket);
     // Test hook: start three bits coarse so large layers exercise the
     // bucket-overflow retry path.
     if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && K >= 3) K -= 3;
@@ -131,58 +141,60 @@ class SortedEngine final : public Engine {
     while (true) {
       const uint32_t K1 = K <= kMaxPassBits ? K : (K + 1) / 2;
       const uint32_t K2 = K - K1;
-      const uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 2048));
-      const uint64_t n1 = (1ull << K1) * NB1;
-      uint32_t NC = 1;
-      if (K2) NC = static_cast<uint32_t>(std::clamp<uint64_t>((Cest >> K1) / 262144 + 1, 1, 16));
-      const uint64_t n2 = K2 ? (1ull << K) * NC : 0;
-      ensure_u32(counts1_, n1);
-      ensure_u32(scan1_, n1 + 1);
-      if (K2) {
-        ensure_u32(counts2_, n2);
-        ensure_u32(scan2_, n2 + 1);
+      uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 2048));
+      // Direct (K2 = 0): the lanes are the sub-buckets, one slab each.
+      // Otherwise every lane part receives the children of NB1 / P blocks
+      // (NB1 is a multiple of P), i.e. C / (lanes * P) on average.
+      const uint32_t P = K2 ? std::min<uint32_t>(kLaneParts, NB1) : 1;
+      NB1 = (NB1 + P - 1) / P * P;
+      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(C) / ((1ull << K1) * P), slack_) : kUniqueCap;
+      const uint64_t lslabs = (1ull << K1) * P;
+      if (!ensure_slabs(K2 ? lslabs * lcap : 0, (1ull << K) * kSlabStride)) {
+        *fatal = kErrFutureCap;
+        terminal = true;
+        return MGBFS_RESOURCE;
       }
-      ensure_u32(scan_ws_, scan_workspace_words(std::max<uint64_t>({n1, n2, 1ull << kMaxBucketBits})) + 16);
+      ensure_u32(lcount_, lslabs);
+      ensure_u32(scount_, 1ull << K);
+      ensure_u32(scan_ws_, scan_workspace_words(1ull << kMaxBucketBits) + 16);
       const uint32_t Ko = std::min<uint32_t>(kMaxBucketBits, K + margin_);
       MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 2, stream_));
+      MGBFS_HIP(fill_u32(scount_.as<uint32_t>(), 0u, 1ull << K, stream_));
+      if (K2) MGBFS_HIP(fill_u32(lcount_.as<uint32_t>(), 0u, lslabs, stream_));
       MGBFS_HIP(hipEventRecord(ev_[0], stream_));
-      const bool all_tagged = layout_.prune && layers.size() > 1;
-      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, all_tagged, K1, NB1,
-                                    counts1_.as<uint32_t>(),
-                                    scan1_.as<uint32_t>(), scan_ws_.as<uint32_t>(), cand1_.as<uint64_t>(), stream_));
+      uint64_t* sub = sub_.as<uint64_t>();
+      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, all_tagged, K1, NB1, P, lcap,
+                                    K2 ? lcap : kSlabStride, K2 ? kErrLaneSlab : kErrBucketCap,
+                                    K2 ? lcount_.as<uint32_t>() : scount_.as<uint32_t>(), error_word(),
+                                    K2 ? lane_.as<uint64_t>() : sub, stream_));
       MGBFS_HIP(hipEventRecord(ev_[1], stream_));
-      uint64_t* seg = cand1_.as<uint64_t>();
-      const uint32_t* segscan = scan1_.as<uint32_t>();
-      uint32_t segNC = NB1;
-      if (K2) {
-        MGBFS_HIP(sorted_partition(cand1_.as<uint64_t>(), scan1_.as<uint32_t>(), NB1, K1, K2, NC,
-                                   counts2_.as<uint32_t>(), scan2_.as<uint32_t>(), scan_ws_.as<uint32_t>(),
-                                   cand2_.as<uint64_t>(), stream_));
-        seg = cand2_.as<uint64_t>();
-        segscan = scan2_.as<uint32_t>();
-        segNC = NC;
-      }
+      if (K2)
+        MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, K1, K2,
+                                   scount_.as<uint32_t>(), error_word(), sub, stream_));
       MGBFS_HIP(hipEventRecord(ev_[2], stream_));
       const bool has_prev = prev_ >= 0;
       MGBFS_HIP(sorted_unique_diff(
-          seg, segscan, segNC, K, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), ko_[cur_],
+          sub, scount_.as<uint32_t>(), K, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), ko_[cur_],
           has_prev ? bank_[prev_].as<uint64_t>() : nullptr, has_prev ? off_[prev_].as<uint32_t>() : nullptr,
           has_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(),
           reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()), layout_.tagbits, unique_grid_, stream_));
+      MGBFS_HIP(hipEventRecord(ev_[6], stream_));
       MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), 1ull << Ko, off_[nxt].as<uint32_t>(),
                                    scan_ws_.as<uint32_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[3], stream_));
-      // Readback: unique count, error word, N (last offset), candidate total.
+      // Readback: unique count, error word, N (last offset).
       MGBFS_HIP(hipMemcpyAsync(host_, control_.as<uint64_t>(), 16, hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 2), off_[nxt].as<uint32_t>() + (1ull << Ko), 4,
-                               hipMemcpyDeviceToHost, stream_));
-      MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 3), scan1_.as<uint32_t>() + n1, 4,
                                hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipStreamSynchronize(stream_));
       const uint32_t err = static_cast<uint32_t>(host_[1]);
       const uint64_t N = static_cast<uint32_t>(host_[2]);
-      const uint64_t C = static_cast<uint32_t>(host_[3]);
-      account(0, 1, 2, 3, F, C, N, has_prev ? count_[prev_] : 0, K2 != 0);
+      account(F, C, N, has_prev ? count_[prev_] : 0, K2 != 0);
+      if (err & kErrLaneSlab) {  // lane slab overflow: retry with twice the slack
+        slack_ = slack_ ? 2 * slack_ : 1;
+        ++retries;
+        continue;
+      }
       if (err & kErrBucketCap) {
         if (K < Klimit) {
           ++K;
@@ -208,8 +220,7 @@ class SortedEngine final : public Engine {
         return MGBFS_RESOURCE;
       }
       MGBFS_HIP(hipEventRecord(ev_[4], stream_));
-      MGBFS_HIP(sorted_compact(seg, segscan, segNC, K, Ko, off_[nxt].as<uint32_t>(), bank_[nxt].as<uint64_t>(),
-                               stream_));
+      MGBFS_HIP(sorted_compact(sub, K, Ko, off_[nxt].as<uint32_t>(), bank_[nxt].as<uint64_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[5], stream_));
       MGBFS_HIP(hipEventSynchronize(ev_[5]));
       float ms = 0.f;
@@ -263,17 +274,21 @@ class SortedEngine final : public Engine {
       if (b != cur_ && b != prev_) return b;
     return -1;
   }
-  // Modelled streaming bytes per phase (see the kernel file header).
-  void account(int e0, int e1, int e2, int e3, uint64_t F, uint64_t C, uint64_t N, uint64_t Fp, bool partitioned) {
-    float a = 0, b = 0, c = 0;
-    MGBFS_HIP(hipEventElapsedTime(&a, ev_[e0], ev_[e1]));
-    MGBFS_HIP(hipEventElapsedTime(&b, ev_[e1], ev_[e2]));
-    MGBFS_HIP(hipEventElapsedTime(&c, ev_[e2], ev_[e3]));
+  // Phase times (events on the engine stream) and modelled streaming bytes
+  // (see the kernel file header).  Phase 2 is unique_diff alone; the fine
+  // count scan is charged to phase 3 with the compaction it feeds.
+  void account(uint64_t F, uint64_t C, uint64_t N, uint64_t Fp, bool partitioned) {
+    float a = 0, b = 0, c = 0, d = 0;
+    MGBFS_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+    MGBFS_HIP(hipEventElapsedTime(&b, ev_[1], ev_[2]));
+    MGBFS_HIP(hipEventElapsedTime(&c, ev_[2], ev_[6]));
+    MGBFS_HIP(hipEventElapsedTime(&d, ev_[6], ev_[3]));
     phase_seconds[0] += a * 1e-3;
     phase_seconds[1] += b * 1e-3;
     phase_seconds[2] += c * 1e-3;
-    kernel_seconds += (a + b + c) * 1e-3;
-    const uint64_t be = 16 * F + 8 * C, bp = partitioned ? 24 * C : 0, bu = 8 * C + 8 * F + 8 * Fp + 8 * N;
+    phase_seconds[3] += d * 1e-3;
+    kernel_seconds += (a + b + c + d) * 1e-3;
+    const uint64_t be = 8 * F + 8 * C, bp = partitioned ? 16 * C : 0, bu = 8 * C + 8 * F + 8 * Fp + 8 * N;
     phase_bytes[0] += be;
     phase_bytes[1] += bp;
     phase_bytes[2] += bu;
@@ -286,17 +301,22 @@ class SortedEngine final : public Engine {
       b.allocate(std::max<uint64_t>(words * 4, b.bytes() * 2));
     }
   }
-  bool ensure_candidates(uint64_t c) {
-    if (cand_cap_ >= c) return true;
-    uint64_t want = std::max<uint64_t>(c, cand_cap_ + cand_cap_ / 2);
-    if (!fits(2 * (want - cand_cap_) * 8)) want = c;
-    if (!fits(2 * (want - cand_cap_) * 8)) return false;
+  // Lane slabs (entries) and sub-bucket slabs (entries); grown, never shrunk.
+  bool ensure_slabs(uint64_t lane_entries, uint64_t sub_entries) {
+    if (lane_cap_ >= lane_entries && sub_cap_ >= sub_entries) return true;
+    uint64_t lane = std::max(lane_cap_, lane_entries), sub = std::max(sub_cap_, sub_entries);
+    if (!fits((lane - lane_cap_ + sub - sub_cap_) * 8)) return false;
     MGBFS_HIP(hipStreamSynchronize(stream_));
-    cand1_.release();
-    cand2_.release();
-    cand1_.allocate(want * 8);
-    cand2_.allocate(want * 8);
-    cand_cap_ = want;
+    if (lane > lane_cap_) {
+      lane_.release();
+      lane_.allocate(lane * 8);
+      lane_cap_ = lane;
+    }
+    if (sub > sub_cap_) {
+      sub_.release();
+      sub_.allocate(sub * 8);
+      sub_cap_ = sub;
+    }
     update_bytes();
     return true;
   }
@@ -323,21 +343,21 @@ class SortedEngine final : public Engine {
     for (uint64_t i = 0; i < count; ++i) out[i] = keyunmix(raw[i] >> layout_.tagbits, layout_.km);
   }
   void update_bytes() {
-    device_bytes = control_.bytes() + fine_counts_.bytes() + cand1_.bytes() + cand2_.bytes() + counts1_.bytes() +
-                   scan1_.bytes() + counts2_.bytes() + scan2_.bytes() + scan_ws_.bytes();
+    device_bytes = control_.bytes() + fine_counts_.bytes() + lane_.bytes() + sub_.bytes() + lcount_.bytes() +
+                   scount_.bytes() + scan_ws_.bytes();
     for (int b = 0; b < 3; ++b) device_bytes += bank_[b].bytes() + off_[b].bytes();
   }
 
   PackedGenerators gens_;
   SortedLayout layout_{};
-  uint32_t n_, margin_ = 0, flags_ = 0, unique_grid_ = 2048;
-  uint64_t start_key_ = 0, budget_ = 0, cap_limit_ = 0, cand_cap_ = 0;
-  DeviceBuffer bank_[3], off_[3], cand1_, cand2_, counts1_, scan1_, counts2_, scan2_, scan_ws_, fine_counts_, control_;
+  uint32_t n_, margin_ = 0, flags_ = 0, unique_grid_ = 2048, slack_ = 1;
+  uint64_t start_key_ = 0, budget_ = 0, cap_limit_ = 0, lane_cap_ = 0, sub_cap_ = 0;
+  DeviceBuffer bank_[3], off_[3], lane_, sub_, lcount_, scount_, scan_ws_, fine_counts_, control_;
   uint64_t bank_cap_[3] = {0, 0, 0}, count_[3] = {0, 0, 0};
   uint32_t ko_[3] = {0, 0, 0};
   int cur_ = 0, prev_ = -1;
   uint64_t* host_ = nullptr;
-  hipEvent_t ev_[6] = {};
+  hipEvent_t ev_[7] = {};
 };
 
 Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o) { return new SortedEngine(g, o); }

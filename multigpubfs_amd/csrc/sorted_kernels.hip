@@ -1,20 +1,27 @@
 // Sort-unique backend with three rotating VRAM banks (inverse-closed sets).
 //
-// States are stored as h = mix64(key) (a bijection, so equality of h is
-// full-key equality).  A bank holds one BFS layer, partitioned by the top
-// Ko bits of h ("fine buckets"), with an offset table of 2^Ko + 1 entries.
-// One layer d -> d+1:
-//   expand_count / expand_scatter : children of bank[d], bucketed by the top
-//        K1 bits into owner lanes (LDS counting sort per 4096-element tile,
-//        contiguous run writes).                              8F + 8F + 8C
-//   partition_count / _scatter    : each lane split by the next K2 bits.
-//                                                             8C + 16C
-//   unique_diff                   : one workgroup per sub-bucket (K = K1+K2
-//        bits): LDS open-addressing set of the candidates (in-layer dedup),
-//        probe with the same sub-bucket of bank[d] and bank[d-1] (exact
-//        three-bank retention for inverse-closed generators), survivors
-//        regrouped by Ko fine bits and written back in place. 8C + 8F + 8F' + 8N
-//   compact                       : survivors -> bank[d+1].   16N
+// States are stored as h = (keymix(key) << tagbits) | tag (keymix is a
+// bijection, so equality of the high bits is full-key equality).  A bank
+// holds one BFS layer, partitioned by the top Ko bits of h ("fine
+// buckets"), with an offset table of 2^Ko + 1 entries.  One layer d -> d+1:
+//   expand_scatter    : children of bank[d], bucketed by the top K1 bits into
+//        owner lanes (LDS counting sort per 8192-element tile).  Each tile
+//        reserves its run in a lane slab with one atomic per lane; every
+//        lane is split into P parts (block b uses part b % P) so no cursor
+//        sees more than 1/P of the tiles.                    8F + 8C
+//   partition_scatter : each lane part split by the next K2 bits into
+//        sub-bucket slabs of kUniqueCap entries (atomic run reservation).
+//                                                             16C
+//   unique_diff       : one workgroup per sub-bucket (K = K1+K2 bits): LDS
+//        open-addressing set of the candidates (in-layer dedup), probe with
+//        the same sub-bucket of bank[d] and bank[d-1] (exact three-bank
+//        retention for inverse-closed generators), survivors regrouped by Ko
+//        fine bits and written back in place.             8C + 8F + 8F' + 8N
+//   compact           : survivors -> bank[d+1].              16N
+// A run that would overflow its slab is dropped and flagged; the host
+// retries the layer with more slab room (lanes) or one more bucket bit
+// (sub-buckets).  Order inside a slab depends on atomic order; the layer
+// sets do not.
 // Replaces the reference's sorted-run owner (cuda/segmented_hash_batch.cu:
 // 157-215: merge sort by (hash,state), binary-search probe of bank runs,
 // exclusive-scan compaction) and its batch owner routing
@@ -92,74 +99,66 @@ __device__ __forceinline__ uint32_t parent_skip(uint64_t v, const SortedLayout& 
   return L.prune ? static_cast<uint32_t>(v & L.tagmask) : 0u;
 }
 
-template <int G>
-__global__ __launch_bounds__(kRouteThreads) void expand_count_kernel(const uint64_t* __restrict__ bank, uint64_t F,
-                                                                      PackedGenerators gens, SortedLayout L,
-                                                                      uint32_t K1, uint32_t NB,
-                                                                      uint32_t* __restrict__ counts) {
-  __shared__ uint32_t hist[1u << kMaxPassBits];
-  const uint32_t buckets = 1u << K1;
-  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) hist[i] = 0;
-  __syncthreads();
-  const uint64_t p0 = F * blockIdx.x / NB, p1 = F * (blockIdx.x + 1) / NB;
-  for (uint64_t base = p0 + threadIdx.x; base < p1; base += kPrefetch * kRouteThreads) {
-    uint64_t pv[kPrefetch];
-#pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
-      const uint64_t p = base + static_cast<uint64_t>(k) * kRouteThreads;
-      pv[k] = p < p1 ? bank[p] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
-      if (base + static_cast<uint64_t>(k) * kRouteThreads >= p1) break;
-      const uint64_t v = pv[k];
-      const uint64_t key = keyunmix(v >> L.tagbits, L.km);
-      const uint32_t skip = parent_skip(v, L);
-#pragma unroll
-      for (int g = 0; g < G; ++g)
-        if (static_cast<uint32_t>(g) + 1 != skip) atomicAdd(&hist[top_bits(child_value(key, gens, g, L), K1)], 1u);
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) counts[static_cast<uint64_t>(i) * NB + blockIdx.x] = hist[i];
-}
-
 // Shared tile machinery.  Each thread holds up to PER tile elements in
 // registers (bit k of `valid` marks v[k]); they are ranked within their
-// bucket by LDS atomics, placed bucket-contiguously in t.s, and written out
-// as contiguous runs at out[running[b] + offset within the tile's run].
+// bucket by LDS atomics and placed bucket-contiguously in t.s.  reserve(b, n)
+// returns the offset of bucket b's run of n elements inside its slab, which
+// starts at slab_base(b) (kDropRun: the slab is full and the run is
+// dropped); the run is then written contiguously.  Run offsets stay 32-bit
+// so two 512-thread tiles fit one CU's LDS.
+constexpr uint32_t kDropRun = ~0u;
+
 struct TileLds {
   uint64_t s[kTile];
+  uint32_t off[1u << kMaxPassBits];
   uint32_t lhist[1u << kMaxPassBits];
   uint32_t lstart[1u << kMaxPassBits];
-  uint32_t running[1u << kMaxPassBits];
   uint32_t wave_sums[kRouteThreads / kWave];
 };
 
-template <int PER, typename BucketFn>
+template <int PER, typename BucketFn, typename ReserveFn, typename BaseFn>
 __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid, uint32_t buckets,
-                                           BucketFn bkt, uint64_t* __restrict__ out) {
+                                           BucketFn bkt, ReserveFn reserve, BaseFn slab_base,
+                                           uint64_t* __restrict__ out) {
   uint32_t rank[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) rank[k] = ((valid >> k) & 1u) ? atomicAdd(&t.lhist[bkt(v[k])], 1u) : 0u;
   __syncthreads();
   lds_exclusive_scan(t.lhist, t.lstart, buckets, t.wave_sums);
   const uint32_t ne = t.lstart[buckets - 1] + t.lhist[buckets - 1];
+  // Run reservations first: their round trips overlap the LDS placement.
+  static_assert((1u << kMaxPassBits) <= 2 * kRouteThreads, "two buckets per thread");
+  const uint32_t b0 = threadIdx.x, b1 = threadIdx.x + kRouteThreads;
+  const uint32_t off0 = b0 < buckets ? reserve(b0, t.lhist[b0]) : kDropRun;
+  const uint32_t off1 = b1 < buckets ? reserve(b1, t.lhist[b1]) : kDropRun;
 #pragma unroll
   for (int k = 0; k < PER; ++k)
     if ((valid >> k) & 1u) t.s[t.lstart[bkt(v[k])] + rank[k]] = v[k];
+  if (b0 < buckets) t.off[b0] = off0;
+  if (b1 < buckets) t.off[b1] = off1;
   __syncthreads();
   for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
     const uint64_t h = t.s[j];
     const uint32_t b = bkt(h);
-    out[static_cast<uint64_t>(t.running[b]) + (j - t.lstart[b])] = h;
+    const uint32_t off = t.off[b];
+    if (off != kDropRun) out[slab_base(b) + off + (j - t.lstart[b])] = h;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) {
-    t.running[i] += t.lhist[i];
-    t.lhist[i] = 0;
-  }
+  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
   __syncthreads();
+}
+
+// Reserves n entries in the slab counted by *cursor (capacity cap); returns
+// the run's offset in the slab, or kDropRun (and sets err_bit) on overflow.
+__device__ __forceinline__ uint32_t reserve_run(uint32_t* __restrict__ cursor, uint32_t cap, uint32_t n,
+                                                uint32_t* __restrict__ error, uint32_t err_bit) {
+  if (n == 0) return kDropRun;
+  const uint32_t off = atomicAdd(cursor, n);
+  if (static_cast<uint64_t>(off) + n > cap) {
+    atomicOr(error, err_bit);
+    return kDropRun;
+  }
+  return off;
 }
 
 // TAGGED: every parent carries a tag (all layers after the start of an
@@ -177,32 +176,41 @@ struct ExpandTile {
 template <int G, bool TAGGED>
 __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uint64_t* __restrict__ bank, uint64_t F,
                                                                         PackedGenerators gens, SortedLayout L,
-                                                                        uint32_t K1, uint32_t NB,
-                                                                        const uint32_t* __restrict__ offsets,
+                                                                        uint32_t K1, uint32_t NB, uint32_t P,
+                                                                        uint32_t cap, uint32_t stride,
+                                                                        uint32_t err_bit,
+                                                                        uint32_t* __restrict__ cursors,
+                                                                        uint32_t* __restrict__ error,
                                                                         uint64_t* __restrict__ out) {
   using T = ExpandTile<G, TAGGED>;
   static_assert(T::kRegs <= 32, "valid mask is 32 bits");
   __shared__ TileLds t;
   const uint32_t buckets = 1u << K1;
-  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) {
-    t.running[i] = offsets[static_cast<uint64_t>(i) * NB + blockIdx.x];
-    t.lhist[i] = 0;
-  }
+  const uint32_t part = blockIdx.x % P;
+  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
   __syncthreads();
-  const uint64_t p0 = F * blockIdx.x / NB, p1 = F * (blockIdx.x + 1) / NB;
+  auto reserve = [=](uint32_t b, uint32_t n) {
+    return reserve_run(&cursors[static_cast<uint64_t>(b) * P + part], cap, n, error, err_bit);
+  };
+  auto slab_base = [=](uint32_t b) { return (static_cast<uint64_t>(b) * P + part) * stride; };
   auto bkt = [K1](uint64_t h) { return top_bits(h, K1); };
-  for (uint64_t t0 = p0; t0 < p1; t0 += T::kParents) {
+  // Tiles of contiguous parents go to blocks round-robin (tile i to block
+  // i mod NB), so every lane part samples the whole frontier.
+  const uint64_t tiles = (F + T::kParents - 1) / T::kParents;
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += NB) {
+    const uint64_t t0 = tile * T::kParents;
     uint64_t pv[T::kParentsPerThread];
+    bool live[T::kParentsPerThread];
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
       const uint64_t q = t0 + static_cast<uint64_t>(m) * kRouteThreads + threadIdx.x;
-      pv[m] = q < p1 ? bank[q] : 0;
+      live[m] = q < F;
+      pv[m] = live[m] ? bank[q] : 0;
     }
     uint64_t v[T::kRegs];
     uint32_t valid = 0;
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
-      const bool live = t0 + static_cast<uint64_t>(m) * kRouteThreads + threadIdx.x < p1;
       const uint64_t key = keyunmix(pv[m] >> L.tagbits, L.km);
       const uint32_t skip = parent_skip(pv[m], L);
       if constexpr (T::kCompact) {
@@ -212,65 +220,43 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
         for (int g = 0; g < G; ++g) c[g] = child_value(key, gens, g, L);
 #pragma unroll
         for (int j = 0; j < G - 1; ++j) v[m * (G - 1) + j] = static_cast<uint32_t>(j) + 1 < skip ? c[j] : c[j + 1];
-        if (live) valid |= ((1u << (G - 1)) - 1u) << (m * (G - 1));
+        if (live[m]) valid |= ((1u << (G - 1)) - 1u) << (m * (G - 1));
       } else {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           v[m * G + g] = child_value(key, gens, g, L);
-          if (live && static_cast<uint32_t>(g) + 1 != skip) valid |= 1u << (m * G + g);
+          if (live[m] && static_cast<uint32_t>(g) + 1 != skip) valid |= 1u << (m * G + g);
         }
       }
     }
-    route_regs(t, v, valid, buckets, bkt, out);
+    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
   }
 }
 
 // ------------------------------------------------------------ partition
 
-__global__ __launch_bounds__(kRouteThreads) void partition_count_kernel(const uint64_t* __restrict__ in,
-                                                                         const uint32_t* __restrict__ scan1, uint32_t NB1,
-                                                                         uint32_t K1, uint32_t K2, uint32_t NC,
-                                                                         uint32_t* __restrict__ counts) {
-  __shared__ uint32_t hist[1u << kMaxPassBits];
-  const uint32_t buckets = 1u << K2;
-  for (uint32_t i = threadIdx.x; i < buckets; i += blockDim.x) hist[i] = 0;
-  __syncthreads();
-  const uint32_t j = blockIdx.x / NC, c = blockIdx.x % NC;
-  const uint64_t gs = scan1[static_cast<uint64_t>(j) * NB1], ge = scan1[static_cast<uint64_t>(j + 1) * NB1];
-  const uint64_t r0 = gs + (ge - gs) * c / NC, r1 = gs + (ge - gs) * (c + 1) / NC;
-  for (uint64_t base = r0 + threadIdx.x; base < r1; base += kPrefetch * kRouteThreads) {
-    uint64_t v[kPrefetch];
-#pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
-      const uint64_t i = base + static_cast<uint64_t>(k) * kRouteThreads;
-      v[k] = i < r1 ? in[i] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kPrefetch; ++k)
-      if (base + static_cast<uint64_t>(k) * kRouteThreads < r1) atomicAdd(&hist[mid_bits(v[k], K1, K2)], 1u);
-  }
-  __syncthreads();
-  for (uint32_t b = threadIdx.x; b < buckets; b += blockDim.x)
-    counts[((static_cast<uint64_t>(j) << K2) + b) * NC + c] = hist[b];
-}
-
+// One block per lane part (lane = blockIdx.x / P): its lcount[] elements at
+// slab (lane * P + part) * lcap are routed by the next K2 bits into the
+// sub-bucket slabs ((lane << K2) + b) * kSlabStride.
 __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const uint64_t* __restrict__ in,
-                                                                           const uint32_t* __restrict__ scan1,
-                                                                           uint32_t NB1, uint32_t K1, uint32_t K2,
-                                                                           uint32_t NC,
-                                                                           const uint32_t* __restrict__ offsets,
+                                                                           const uint32_t* __restrict__ lcount,
+                                                                           uint32_t lcap, uint32_t P, uint32_t K1,
+                                                                           uint32_t K2,
+                                                                           uint32_t* __restrict__ scount,
+                                                                           uint32_t* __restrict__ error,
                                                                            uint64_t* __restrict__ out) {
   __shared__ TileLds t;
   const uint32_t buckets = 1u << K2;
-  const uint32_t j = blockIdx.x / NC, c = blockIdx.x % NC;
-  for (uint32_t b = threadIdx.x; b < buckets; b += kRouteThreads) {
-    t.running[b] = offsets[((static_cast<uint64_t>(j) << K2) + b) * NC + c];
-    t.lhist[b] = 0;
-  }
+  for (uint32_t b = threadIdx.x; b < buckets; b += kRouteThreads) t.lhist[b] = 0;
   __syncthreads();
-  const uint64_t gs = scan1[static_cast<uint64_t>(j) * NB1], ge = scan1[static_cast<uint64_t>(j + 1) * NB1];
-  const uint64_t r0 = gs + (ge - gs) * c / NC, r1 = gs + (ge - gs) * (c + 1) / NC;
+  const uint64_t slab = blockIdx.x;
+  const uint64_t r0 = slab * lcap, r1 = r0 + min(lcount[slab], lcap);
   auto bkt = [K1, K2](uint64_t h) { return mid_bits(h, K1, K2); };
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x / P) << K2;  // first sub-bucket of this lane
+  auto reserve = [=](uint32_t b, uint32_t n) {
+    return reserve_run(&scount[first + b], kUniqueCap, n, error, kErrBucketCap);
+  };
+  auto slab_base = [=](uint32_t b) { return (first + b) * kSlabStride; };
   for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
     uint64_t v[kTilePerThread];
     uint32_t valid = 0;
@@ -280,19 +266,14 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
       v[k] = i < r1 ? in[i] : 0;
       if (i < r1) valid |= 1u << k;
     }
-    route_regs(t, v, valid, buckets, bkt, out);
+    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
   }
 }
 
 // ------------------------------------------------------ unique + difference
 
-__device__ __forceinline__ bool same_state(uint64_t a, uint64_t b, uint32_t tagbits) {
-  return ((a ^ b) >> tagbits) == 0;
-}
-
 struct UniqueLds {
-  uint64_t table[kUniqueSlots];
-  uint32_t found[kUniqueSlots / 32];
+  alignas(16) uint64_t table[kUniqueSlots];  // read two slots at a time
   uint32_t fhist[1u << kMaxFineBits];
   uint32_t fstart[1u << kMaxFineBits];
   uint32_t wave_sums[kUniqueThreads / kWave];
@@ -300,29 +281,43 @@ struct UniqueLds {
   uint32_t unique;
 };
 
+struct UniqueArgs {
+  uint64_t* cand;          // sub-bucket slabs of kUniqueCap entries
+  const uint32_t* scount;  // entries reserved per slab (> kUniqueCap: overflowed)
+  const uint64_t* cur;
+  const uint32_t* cur_off;
+  const uint64_t* prev;
+  const uint32_t* prev_off;
+  uint32_t* fine_counts;
+  uint32_t* error;
+  unsigned long long* unique_total;
+  uint64_t tagmask;  // (1 << tagbits) - 1
+  uint64_t mark;     // tag value no candidate carries: "present in a bank"
+  uint32_t K, Kc, Kp, Ko, tagbits, buckets;
+};
+
 constexpr uint64_t kEmpty = ~0ull;
 
-__device__ __forceinline__ int lds_find(const UniqueLds& u, uint64_t h, uint32_t tagbits, uint32_t mask) {
-  uint32_t slot = static_cast<uint32_t>(h >> tagbits) & mask;
-  for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const uint64_t v = u.table[slot];
-    if (v == kEmpty) return -1;
-    if (same_state(v, h, tagbits)) return static_cast<int>(slot);
-    slot = (slot + 1) & mask;
-  }
-  return -1;
-}
-
-// Marks table entries present in bank[lo, hi).  The first kPrefetch
-// elements of this thread (pre) were loaded before the table was built;
-// later chunks are loaded in batches.
-__device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, uint32_t tagbits, uint32_t mask) {
+// Bank entry h: if its state is in the table, overwrite that slot's tag
+// with `mark`.  Each slot matches at most one bank entry (the two banks are
+// disjoint layers of unique states), so a plain store suffices; concurrent
+// probes passing the slot see the same state bits either way.
+__device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, const UniqueArgs& a, uint32_t mask) {
   if (h == kEmpty) {
     atomicOr(&u.flags, 2u);
     return;
   }
-  const int s = lds_find(u, h, tagbits, mask);
-  if (s >= 0) atomicOr(&u.found[s >> 5], 1u << (s & 31));
+  const uint64_t key = h | a.tagmask;
+  uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const uint64_t v = u.table[slot];
+    if (v == kEmpty) return;
+    if ((v | a.tagmask) == key) {
+      u.table[slot] = (v & ~a.tagmask) | a.mark;
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
 }
 
 __device__ __forceinline__ void prefetch_slice(const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
@@ -335,10 +330,10 @@ __device__ __forceinline__ void prefetch_slice(const uint64_t* __restrict__ bank
 }
 
 __device__ __forceinline__ void probe_bank(UniqueLds& u, const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
-                                           const uint64_t (&pre)[kPrefetch], uint32_t tagbits, uint32_t mask) {
+                                           const uint64_t (&pre)[kPrefetch], const UniqueArgs& a, uint32_t mask) {
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k)
-    if (lo + threadIdx.x + static_cast<uint64_t>(k) * kUniqueThreads < hi) probe_value(u, pre[k], tagbits, mask);
+    if (lo + threadIdx.x + static_cast<uint64_t>(k) * kUniqueThreads < hi) probe_value(u, pre[k], a, mask);
   for (uint64_t base = lo + threadIdx.x + kPrefetch * kUniqueThreads; base < hi; base += kPrefetch * kUniqueThreads) {
     uint64_t v[kPrefetch];
 #pragma unroll
@@ -348,34 +343,23 @@ __device__ __forceinline__ void probe_bank(UniqueLds& u, const uint64_t* __restr
     }
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k)
-      if (base + static_cast<uint64_t>(k) * kUniqueThreads < hi) probe_value(u, v[k], tagbits, mask);
+      if (base + static_cast<uint64_t>(k) * kUniqueThreads < hi) probe_value(u, v[k], a, mask);
   }
 }
 
-// Bounds of sub-bucket f: candidates [s, e), current-bank slice [clo, chi),
-// previous-bank slice [plo, phi).
+// Bounds of sub-bucket f: candidates [s, e) of its slab, current-bank slice
+// [clo, chi), previous-bank slice [plo, phi).
 struct BucketBounds {
-  uint32_t s, e, clo, chi, plo, phi;
+  uint64_t s, e;
+  uint32_t clo, chi, plo, phi;
 };
 
-struct UniqueArgs {
-  uint64_t* cand;
-  const uint32_t* cscan;
-  const uint64_t* cur;
-  const uint32_t* cur_off;
-  const uint64_t* prev;
-  const uint32_t* prev_off;
-  uint32_t* fine_counts;
-  uint32_t* error;
-  unsigned long long* unique_total;
-  uint32_t NC, K, Kc, Kp, Ko, tagbits, buckets;
-};
 
 __device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_t f) {
   BucketBounds b{0, 0, 0, 0, 0, 0};
   if (f >= a.buckets) return b;
-  b.s = a.cscan[static_cast<uint64_t>(f) * a.NC];
-  b.e = a.cscan[static_cast<uint64_t>(f + 1) * a.NC];
+  b.s = static_cast<uint64_t>(f) * kSlabStride;
+  b.e = b.s + a.scount[f];
   if (a.cur) {
     b.clo = a.cur_off[static_cast<uint64_t>(f) << (a.Kc - a.K)];
     b.chi = a.cur_off[static_cast<uint64_t>(f + 1) << (a.Kc - a.K)];
@@ -396,128 +380,120 @@ __device__ __forceinline__ void load_data(const UniqueArgs& a, const BucketBound
   const bool fits = b.e - b.s <= kUniqueCap;
 #pragma unroll
   for (int k = 0; k < kUniquePerThread; ++k) {
-    const uint32_t i = b.s + k * kUniqueThreads + threadIdx.x;
+    const uint64_t i = b.s + k * kUniqueThreads + threadIdx.x;
     d.v[k] = (fits && i < b.e) ? a.cand[i] : 0;
   }
   prefetch_slice(a.cur, b.clo, b.chi, d.cpre);
   prefetch_slice(a.prev, b.plo, b.phi, d.ppre);
 }
 
-// Persistent blocks: block x handles sub-buckets x, x + grid, ...  Loads run
-// two stages ahead of the LDS work: bounds of bucket i+2 and candidates /
-// first bank chunks of bucket i+1 are in flight while bucket i is deduped.
-// Each thread keeps its (at most kUniquePerThread) candidates and their
-// slots in registers, so only the candidates it inserted first are revisited.
+constexpr uint32_t kNoSlot = ~0u;
+
+// Block x handles sub-buckets x, x + grid, ... (the engine launches one
+// block per sub-bucket).  The candidates and the first bank chunks are
+// loaded before the LDS table is built.  Phases: insert every candidate
+// (in-layer dedup; state equality ignores tags), each thread remembering
+// the slots it claimed -> probe both banks, marking matched slots -> the
+// claimed, unmarked slots are the survivors: count them per fine bucket,
+// scan, and write them back into the slab grouped by fine bucket.
 __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void unique_diff_kernel(
     UniqueArgs a) {
   __shared__ UniqueLds u;
   const uint32_t fine_bits = a.Ko - a.K, fine = 1u << fine_bits;
-  const uint32_t step = gridDim.x;
-  uint32_t f = blockIdx.x;
-  BucketBounds bcur = load_bounds(a, f);
-  BucketBounds bnext = load_bounds(a, f + step);
-  BucketData dcur;
-  load_data(a, bcur, dcur);
   unsigned long long unique_block = 0;
-  for (; f < a.buckets; f += step) {
-    const BucketBounds bafter = load_bounds(a, f + 2 * step);
-    BucketData dnext;
-    load_data(a, bnext, dnext);
-
-    const uint32_t s = bcur.s, e = bcur.e;
-    const uint32_t c = e - s;
+  for (uint32_t f = blockIdx.x; f < a.buckets; f += gridDim.x) {
+    const BucketBounds bcur = load_bounds(a, f);
+    BucketData dcur;
+    load_data(a, bcur, dcur);
+    const uint64_t s = bcur.s, e = bcur.e;
+    const uint64_t c = e - s;
     if (c > kUniqueCap) {
       if (threadIdx.x == 0) atomicOr(a.error, kErrBucketCap);
       for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads)
         a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = 0;
-    } else {
-      // Table sized to the bucket: slots = next_pow2(2c) (load factor <= 0.5).
-      uint32_t slots = 64;
-      while (slots < 2 * c) slots <<= 1;
-      const uint32_t mask = slots - 1;
-      for (uint32_t i = threadIdx.x; i < slots; i += kUniqueThreads) u.table[i] = kEmpty;
-      for (uint32_t i = threadIdx.x; i < slots / 32 + 1; i += kUniqueThreads) u.found[i] = 0;
-      for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
-      if (threadIdx.x == 0) {
-        u.flags = 0;
-        u.unique = 0;
-      }
-      __syncthreads();
-      // In-layer dedup: insert every candidate (state equality ignores tags).
-      uint32_t slot_of[kUniquePerThread];
-      uint32_t owned = 0;  // bit k: this thread inserted v[k] first
-      uint32_t special = 0;
-#pragma unroll
-      for (int k = 0; k < kUniquePerThread; ++k) {
-        slot_of[k] = 0;
-        if (s + k * kUniqueThreads + threadIdx.x >= e) continue;
-        const uint64_t h = dcur.v[k];
-        if (h == kEmpty) {
-          if (!(atomicOr(&u.flags, 1u) & 1u)) special = 1;
-          continue;
-        }
-        uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
-        while (true) {
-          const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
-          if (old == kEmpty) {
-            owned |= 1u << k;
-            slot_of[k] = slot;
-            break;
-          }
-          if (same_state(old, h, a.tagbits)) break;
-          slot = (slot + 1) & mask;
-        }
-      }
-      const uint32_t mine = __popc(owned) + special;
-      if (mine) atomicAdd(&u.unique, mine);
-      __syncthreads();
-      // Difference against the two retained layers (same sub-bucket only).
-      probe_bank(u, a.cur, bcur.clo, bcur.chi, dcur.cpre, a.tagbits, mask);
-      probe_bank(u, a.prev, bcur.plo, bcur.phi, dcur.ppre, a.tagbits, mask);
-      __syncthreads();
-      const bool max_survives = special && !(u.flags & 2u);
-      uint32_t survive = 0;
-#pragma unroll
-      for (int k = 0; k < kUniquePerThread; ++k) {
-        if ((owned >> k) & 1u) {
-          const uint32_t sl = slot_of[k];
-          if (!((u.found[sl >> 5] >> (sl & 31)) & 1u)) {
-            survive |= 1u << k;
-            atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u);
-          }
-        }
-      }
-      if (max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
-      __syncthreads();
-      lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
-      for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) {
-        a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = u.fhist[i];
-        u.fhist[i] = u.fstart[i];
-      }
-      __syncthreads();
-      // Every candidate is in registers: write survivors back in place.
-#pragma unroll
-      for (int k = 0; k < kUniquePerThread; ++k)
-        if ((survive >> k) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u)] = dcur.v[k];
-      if (max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
-      if (threadIdx.x == 0) unique_block += u.unique;
-      __syncthreads();  // LDS reuse by the next bucket
+      continue;
     }
-    bcur = bnext;
-    bnext = bafter;
-    dcur = dnext;
+    // Table sized to the bucket: slots = next_pow2(2c) (load factor <= 0.5).
+    uint32_t slots = 64;
+    while (slots < 2 * c) slots <<= 1;
+    const uint32_t mask = slots - 1;
+    ulonglong2* table2 = reinterpret_cast<ulonglong2*>(u.table);
+    for (uint32_t i = threadIdx.x; i < slots / 2; i += kUniqueThreads) table2[i] = make_ulonglong2(kEmpty, kEmpty);
+    for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
+    if (threadIdx.x == 0) {
+      u.flags = 0;
+      u.unique = 0;
+    }
+    __syncthreads();
+    uint32_t slot_of[kUniquePerThread];
+    uint32_t inserted = 0;
+#pragma unroll
+    for (int k = 0; k < kUniquePerThread; ++k) {
+      slot_of[k] = kNoSlot;
+      if (s + k * kUniqueThreads + threadIdx.x >= e) continue;
+      const uint64_t h = dcur.v[k];
+      if (h == kEmpty) {
+        atomicOr(&u.flags, 1u);
+        continue;
+      }
+      const uint64_t key = h | a.tagmask;
+      uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
+      bool claimed = false;
+      while (true) {
+        const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
+        if (old == kEmpty) {
+          claimed = true;
+          break;
+        }
+        if ((old | a.tagmask) == key) break;
+        slot = (slot + 1) & mask;
+      }
+      if (claimed) {
+        slot_of[k] = slot;
+        ++inserted;
+      }
+    }
+    if (inserted) atomicAdd(&u.unique, inserted);
+    __syncthreads();
+    // Difference against the two retained layers (same sub-bucket only).
+    probe_bank(u, a.cur, bcur.clo, bcur.chi, dcur.cpre, a, mask);
+    probe_bank(u, a.prev, bcur.plo, bcur.phi, dcur.ppre, a, mask);
+    __syncthreads();
+    // The all-ones state cannot live in the table (it is the empty marker).
+    const bool max_survives = (u.flags & 3u) == 1u;
+    uint32_t survive = 0;
+#pragma unroll
+    for (int k = 0; k < kUniquePerThread; ++k)
+      if (slot_of[k] != kNoSlot && (u.table[slot_of[k]] & a.tagmask) != a.mark) {
+        survive |= 1u << k;
+        atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u);
+      }
+    if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
+    __syncthreads();
+    lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
+    for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) {
+      a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = u.fhist[i];
+      u.fhist[i] = u.fstart[i];
+    }
+    __syncthreads();
+    // Every survivor is in registers: write it back in place.
+#pragma unroll
+    for (int k = 0; k < kUniquePerThread; ++k)
+      if ((survive >> k) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u)] = dcur.v[k];
+    if (threadIdx.x == 0 && max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
+    if (threadIdx.x == 0) unique_block += u.unique + (u.flags & 1u);
+    __syncthreads();  // LDS reuse by the next bucket
   }
   if (threadIdx.x == 0 && unique_block) atomicAdd(a.unique_total, unique_block);
 }
 
-// grid = 2^K blocks: survivors of sub-bucket f -> bank rows.
-__global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand,
-                                                                const uint32_t* __restrict__ cscan, uint32_t NC,
-                                                                uint32_t K, uint32_t Ko,
-                                                                const uint32_t* __restrict__ off,
+// grid = 2^K blocks: survivors of sub-bucket f (at the start of its slab)
+// -> bank rows.
+__global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand, uint32_t K,
+                                                                uint32_t Ko, const uint32_t* __restrict__ off,
                                                                 uint64_t* __restrict__ bank) {
   const uint32_t f = blockIdx.x;
-  const uint64_t s = cscan[static_cast<uint64_t>(f) * NC];
+  const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
   const uint32_t sh = Ko - K;
   const uint64_t lo = off[static_cast<uint64_t>(f) << sh], hi = off[static_cast<uint64_t>(f + 1) << sh];
   for (uint64_t i = threadIdx.x; i < hi - lo; i += kRouteThreads) bank[lo + i] = cand[s + i];
@@ -546,26 +522,27 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
 
 template <int G, bool TAGGED>
 static hipError_t expand_route_g(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                                 uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan, uint32_t* scan_ws,
-                                 uint64_t* out, hipStream_t stream) {
-  hipLaunchKernelGGL(expand_count_kernel<G>, dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L, K1, NB,
-                     counts);
-  hipError_t e = exclusive_scan_u32(counts, (static_cast<uint64_t>(1) << K1) * NB, scan, scan_ws, stream);
-  if (e != hipSuccess) return e;
+                                 uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                                 uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
+                                 hipStream_t stream) {
   hipLaunchKernelGGL((expand_scatter_kernel<G, TAGGED>), dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L,
-                     K1, NB, scan, out);
+                     K1, NB, P, cap, stride, err_bit, cursors, error, out);
   return hipGetLastError();
 }
 
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t* counts, uint32_t* scan,
-                               uint32_t* scan_ws, uint64_t* out, hipStream_t stream) {
-  if (K1 > kMaxPassBits || NB == 0) return hipErrorInvalidValue;
+                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
+                               hipStream_t stream) {
+  if (K1 > kMaxPassBits || NB == 0 || P == 0 || cap == 0 || stride < cap) return hipErrorInvalidValue;
   switch (gens.count) {
-#define MGBFS_G(k)                                                                                        \
-  case k:                                                                                                 \
-    return all_tagged ? expand_route_g<k, true>(bank, F, gens, L, K1, NB, counts, scan, scan_ws, out, stream) \
-                      : expand_route_g<k, false>(bank, F, gens, L, K1, NB, counts, scan, scan_ws, out, stream);
+#define MGBFS_G(k)                                                                                               \
+  case k:                                                                                                        \
+    return all_tagged                                                                                            \
+               ? expand_route_g<k, true>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
+                                         stream)                                                            \
+               : expand_route_g<k, false>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
+                                          stream);
     MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
     MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
 #undef MGBFS_G
@@ -574,37 +551,37 @@ hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGen
   }
 }
 
-hipError_t sorted_partition(const uint64_t* in, const uint32_t* scan1, uint32_t NB1, uint32_t K1, uint32_t K2,
-                            uint32_t NC, uint32_t* counts, uint32_t* scan2, uint32_t* scan_ws, uint64_t* out,
-                            hipStream_t stream) {
-  if (K2 == 0 || K2 > kMaxPassBits || K1 + K2 > kMaxBucketBits || NC == 0) return hipErrorInvalidValue;
-  const uint32_t blocks = (1u << K1) * NC;
-  hipLaunchKernelGGL(partition_count_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, scan1, NB1, K1, K2, NC,
-                     counts);
-  hipError_t e = exclusive_scan_u32(counts, (static_cast<uint64_t>(1) << (K1 + K2)) * NC, scan2, scan_ws, stream);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, scan1, NB1, K1, K2,
-                     NC, scan2, out);
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
+                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, hipStream_t stream) {
+  if (K2 == 0 || K2 > kMaxPassBits || K1 + K2 > kMaxBucketBits || P == 0 || lcap == 0) return hipErrorInvalidValue;
+  const uint32_t blocks = (1u << K1) * P;
+  hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, lcount, lcap, P, K1,
+                     K2, scount, error, out);
   return hipGetLastError();
 }
 
-hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, const uint64_t* cur,
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
                               const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, uint32_t tagbits, uint32_t grid_limit,
                               hipStream_t stream) {
   if (K > Ko || Ko - K > kMaxFineBits || (cur && K > Kc) || (prev && K > Kp) || grid_limit == 0)
     return hipErrorInvalidValue;
-  UniqueArgs a{cand, cscan, cur, cur_off, prev, prev_off, fine_counts, error, unique_total,
-               NC, K, Kc, Kp, Ko, tagbits, 1u << K};
+  if (tagbits < 4 || tagbits > 60) return hipErrorInvalidValue;
+  const uint64_t tagmask = (1ull << tagbits) - 1;
+  // Tags in use are 0 and 1..G (G + 1 < tagmask when tags are used), so
+  // tagmask - 1 is never a candidate's tag and never turns a state into
+  // the all-ones empty marker.
+  UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, tagmask, tagmask - 1,
+               K, Kc, Kp, Ko, tagbits, 1u << K};
   const uint32_t grid = std::min<uint32_t>(1u << K, grid_limit);
   hipLaunchKernelGGL(unique_diff_kernel, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t sorted_compact(const uint64_t* cand, const uint32_t* cscan, uint32_t NC, uint32_t K, uint32_t Ko,
-                          const uint32_t* off, uint64_t* bank, hipStream_t stream) {
-  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, cscan, NC, K, Ko, off, bank);
+hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, uint64_t* bank,
+                          hipStream_t stream) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, K, Ko, off, bank);
   return hipGetLastError();
 }
 

@@ -17,3 +17,16 @@ def test_range_set_matches_occupancy_model(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "arena ok" in out.stdout
+
+
+def test_key_mixer_is_bijective_and_spreads_buckets(tmp_path):
+    """The sorted backend's key mixer (multigpubfs_amd/csrc/common.hpp)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("hipcc not installed")
+    exe = tmp_path / "keymix_check"
+    subprocess.run([hipcc, "-std=c++17", "-O1", "-I", str(ROOT / "multigpubfs_amd" / "csrc"),
+                    str(ROOT / "tests" / "native" / "keymix_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "keymix ok" in out.stdout

@@ -87,6 +87,16 @@ def test_s11_sorted_overflow_retry_path_is_exact():
     assert info["retries"] > 0
 
 
+def test_s11_sorted_lane_slab_retry_path_is_exact():
+    # Owner-lane slabs sized to the mean overflow; the layer is redone with slack.
+    ref = O.packed_bfs(11)["layers"]
+    with NativeEngine(lrx(11), backend="sorted", flags=4) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref
+    assert info["retries"] > 0
+
+
 @pytest.mark.parametrize("backend", BACKENDS)
 @pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
 def test_random_inverse_closed_sets_match_oracle(backend, seed):
