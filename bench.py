@@ -157,7 +157,7 @@ def main(argv=None):
         models = ["8F frontier read + 8N next-layer write (bitmap probes excluded)"]
     else:
         names = ["expand_scatter_kernel", "partition_scatter_kernel", "unique_diff_kernel", "compact_kernel"]
-        models = ["8F + 8C", "16C", "8C + 8F + 8F_prev + 8N", "16N"]
+        models = ["8F + 8C", "16C", "8C + 8F + 8N (back-edge masks: current bank only)", "16N"]
     phases = []
     for i, name in enumerate(names):
         if phase_s[i] <= 0:

@@ -55,6 +55,8 @@ extern "C" {
                                              collides and full-state equality decides */
 #define MGBFS_FLAG_TEST_TIGHT_SLABS 4u    /* sorted backend: owner-lane slabs start with
                                              no slack, so lane overflows take the retry path */
+#define MGBFS_FLAG_TEST_NO_PRUNE 8u       /* sorted backend: no back-edge masks; every
+                                             child is generated and both banks are probed */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are

@@ -23,7 +23,7 @@ constexpr uint32_t kSlabStride = 4096;    // entries between sub-bucket slab sta
 struct SortedLayout {
   KeyMix km;
   uint32_t tagbits;  // 64 - 4n
-  uint32_t prune;    // parent-edge tags in use
+  uint32_t prune;    // back-edge masks in use: tag bit i = skip generator i
   uint64_t tagmask;  // (1 << tagbits) - 1
 };
 
@@ -37,7 +37,7 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
 // cursors[] counts per slab
 // (zeroed by the caller); a run beyond `cap` is dropped and sets err_bit.
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                hipStream_t stream);
 // Lane slabs (lcount per slab, capacity and stride lcap) -> sub-bucket
@@ -45,11 +45,14 @@ hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGen
 // zeroed by the caller).
 hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
                             uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, hipStream_t stream);
+// In-layer dedup of the sub-bucket slabs and difference against the
+// current bank and, when given, the previous one; survivors are written back
+// grouped by Ko bits.  merge: OR duplicates' back-edge masks.
 hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
                               const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
-                              unsigned long long* unique_total, uint32_t tagbits, uint32_t grid_limit,
-                              hipStream_t stream);
+                              unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
+                              bool merge, uint32_t grid_limit, hipStream_t stream);
 hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, uint64_t* bank,
                           hipStream_t stream);
 

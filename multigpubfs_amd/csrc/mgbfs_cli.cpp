@@ -385,7 +385,7 @@ int cmd_graph(const std::vector<std::string>& args) {
   report["devices"] = devices;
   report["state_bytes"] = Json::integer(info.state_bytes);
   report["states_sha256"] = Json::str(mgbfs::host::sha256_hex(states_text));
-  report["edges_expanded"] = Json::unsigned_(info.candidates);
+  report["candidates_generated"] = Json::unsigned_(info.candidates);  // after back-edge pruning
   report["kernel_seconds"] = Json::real(info.kernel_seconds);
   report["retention"] = Json::str("current sample <=1000 and previous layer only if <1000");
   report["scope"] = Json::str("single-device MI355X engine");

@@ -1,4 +1,4 @@
-// Host runtime of the sort-unique backend with three rotating VRAM banks.
+// Host runtime of the sort-unique backend with rotating VRAM banks.
 //
 // Layer loop (one host readback per layer, plus a retry when a slab
 // overflows): expand+route into owner-lane slabs -> partition into
@@ -7,6 +7,11 @@
 // Retention is exact for inverse-closed generator sets: every neighbour of
 // layer d lies in layer d-1, d or d+1 (graph_definition.py:84-92; retention
 // rule history_layers = 3, crates/mgbfs-runtime/src/generic_distributed_run.rs:40).
+// With back-edge masks (G <= tagbits - 2) each stored state carries the set
+// of generators leading back to layer d-1 -- the OR over all the duplicate
+// candidates that discovered it -- and expansion skips them, so every child
+// lies in layer d or d+1 and only the current bank is probed.  Otherwise
+// nothing is skipped and both retained banks are probed.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -65,10 +70,12 @@ class SortedEngine final : public Engine {
     const std::vector<int> inv = inverse_index(gens_);
     for (int x : inv)
       if (x < 0) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
-    // Tags 1..G must leave the all-ones tag value unused (kEmpty sentinel).
-    layout_.prune = layout_.tagbits >= 5 && gens_.count + 1 < (1ull << layout_.tagbits) - 1;
+    // Back-edge masks use tag bits 0..G-1; bit tagbits-1 is the in-bank
+    // mark and bit tagbits-2 stays clear, so no entry is all ones.
+    layout_.prune = gens_.count + 2 <= layout_.tagbits && !(flags_ & MGBFS_FLAG_TEST_NO_PRUNE);
     for (uint32_t a = 0; a < gens_.count; ++a)
-      gens_.child_tag[a] = (layout_.prune && inv[a] >= 0) ? static_cast<uint8_t>(inv[a] + 1) : 0;
+      gens_.child_tag[a] = layout_.prune ? static_cast<uint8_t>(inv[a] + 1) : 0;
+    history_layers = layout_.prune ? 2 : 3;
     margin_ = std::min<uint32_t>(kMaxFineBits, 2 * (ceil_log2(gens_.count) + 1));
     start_key_ = g.start_key();
     // One workgroup per sub-bucket measured faster than a persistent grid
@@ -128,15 +135,18 @@ class SortedEngine final : public Engine {
       return MGBFS_RESOURCE;
     }
     const int nxt = next_bank();
-    // Every parent yields exactly G children, or G-1 when it carries a tag.
-    const bool all_tagged = layout_.prune && layers.size() > 1;
-    const uint64_t C = all_tagged ? F * (G - 1) : Cmax;
-    uint32_t K = C <= kTargetBucket ? 0 : ceil_log2((C + kTargetBucket - 1) / kTargetBucket);
+    // Every parent yields at most G children, at most G-1 when it carries a
+    // back-edge mask (every state but the start does); the slabs and the
+    // bucket bits are sized for that bound.
+    const bool masked = layout_.prune && layers.size() > 1;
+    const uint64_t Cest = masked ? F * (G > 1 ? G - 1 : 1) : Cmax;
+    uint32_t K = Cest <= kTargetBucket ? 0 : ceil_log2((Cest + kTargetBucket - 1) / kTargetBucket);
     // Test hook: start three bits coarse so large layers exercise the
     // bucket-overflow retry path.
     if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && K >= 3) K -= 3;
+    const bool probe_prev = prev_ >= 0 && !layout_.prune;
     uint32_t Klimit = std::min<uint32_t>(kMaxBucketBits, ko_[cur_]);
-    if (prev_ >= 0) Klimit = std::min(Klimit, ko_[prev_]);
+    if (probe_prev) Klimit = std::min(Klimit, ko_[prev_]);
     K = std::min(K, Klimit);
     while (true) {
       const uint32_t K1 = K <= kMaxPassBits ? K : (K + 1) / 2;
@@ -147,7 +157,7 @@ class SortedEngine final : public Engine {
       // (NB1 is a multiple of P), i.e. C / (lanes * P) on average.
       const uint32_t P = K2 ? std::min<uint32_t>(kLaneParts, NB1) : 1;
       NB1 = (NB1 + P - 1) / P * P;
-      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(C) / ((1ull << K1) * P), slack_) : kUniqueCap;
+      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(Cest) / ((1ull << K1) * P), slack_) : kUniqueCap;
       const uint64_t lslabs = (1ull << K1) * P;
       if (!ensure_slabs(K2 ? lslabs * lcap : 0, (1ull << K) * kSlabStride)) {
         *fatal = kErrFutureCap;
@@ -158,12 +168,12 @@ class SortedEngine final : public Engine {
       ensure_u32(scount_, 1ull << K);
       ensure_u32(scan_ws_, scan_workspace_words(1ull << kMaxBucketBits) + 16);
       const uint32_t Ko = std::min<uint32_t>(kMaxBucketBits, K + margin_);
-      MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 2, stream_));
+      MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 3, stream_));
       MGBFS_HIP(fill_u32(scount_.as<uint32_t>(), 0u, 1ull << K, stream_));
       if (K2) MGBFS_HIP(fill_u32(lcount_.as<uint32_t>(), 0u, lslabs, stream_));
       MGBFS_HIP(hipEventRecord(ev_[0], stream_));
       uint64_t* sub = sub_.as<uint64_t>();
-      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, all_tagged, K1, NB1, P, lcap,
+      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, K1, NB1, P, lcap,
                                     K2 ? lcap : kSlabStride, K2 ? kErrLaneSlab : kErrBucketCap,
                                     K2 ? lcount_.as<uint32_t>() : scount_.as<uint32_t>(), error_word(),
                                     K2 ? lane_.as<uint64_t>() : sub, stream_));
@@ -172,24 +182,25 @@ class SortedEngine final : public Engine {
         MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, K1, K2,
                                    scount_.as<uint32_t>(), error_word(), sub, stream_));
       MGBFS_HIP(hipEventRecord(ev_[2], stream_));
-      const bool has_prev = prev_ >= 0;
+      auto* control = reinterpret_cast<unsigned long long*>(control_.as<uint64_t>());
       MGBFS_HIP(sorted_unique_diff(
           sub, scount_.as<uint32_t>(), K, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), ko_[cur_],
-          has_prev ? bank_[prev_].as<uint64_t>() : nullptr, has_prev ? off_[prev_].as<uint32_t>() : nullptr,
-          has_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(),
-          reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()), layout_.tagbits, unique_grid_, stream_));
+          probe_prev ? bank_[prev_].as<uint64_t>() : nullptr, probe_prev ? off_[prev_].as<uint32_t>() : nullptr,
+          probe_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(), control, control + 2,
+          layout_.tagbits, layout_.prune != 0, unique_grid_, stream_));
       MGBFS_HIP(hipEventRecord(ev_[6], stream_));
       MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), 1ull << Ko, off_[nxt].as<uint32_t>(),
                                    scan_ws_.as<uint32_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[3], stream_));
-      // Readback: unique count, error word, N (last offset).
-      MGBFS_HIP(hipMemcpyAsync(host_, control_.as<uint64_t>(), 16, hipMemcpyDeviceToHost, stream_));
-      MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 2), off_[nxt].as<uint32_t>() + (1ull << Ko), 4,
+      // Readback: unique count, error word, candidate count, N (last offset).
+      MGBFS_HIP(hipMemcpyAsync(host_, control_.as<uint64_t>(), 24, hipMemcpyDeviceToHost, stream_));
+      MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 3), off_[nxt].as<uint32_t>() + (1ull << Ko), 4,
                                hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipStreamSynchronize(stream_));
       const uint32_t err = static_cast<uint32_t>(host_[1]);
-      const uint64_t N = static_cast<uint32_t>(host_[2]);
-      account(F, C, N, has_prev ? count_[prev_] : 0, K2 != 0);
+      const uint64_t C = host_[2];
+      const uint64_t N = static_cast<uint32_t>(host_[3]);
+      account(F, C, N, probe_prev ? count_[prev_] : 0, K2 != 0);
       if (err & kErrLaneSlab) {  // lane slab overflow: retry with twice the slack
         slack_ = slack_ ? 2 * slack_ : 1;
         ++retries;

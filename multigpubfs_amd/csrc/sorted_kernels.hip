@@ -87,16 +87,14 @@ __device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t c
 
 // ------------------------------------------------------------ expansion
 
-// Stored value of a child made by generator g from `key`.
+// Stored value of a child made by generator g from `key`.  With back-edge
+// masks the tag is the bit of g's inverse (child_tag[g] = inverse + 1): the
+// child's generator that leads back to this parent.
 __device__ __forceinline__ uint64_t child_value(uint64_t key, const PackedGenerators& gens, int g,
                                                 const SortedLayout& L) {
   const uint64_t child = apply_groups(key, gens, g);
-  return (keymix(child, L.km) << L.tagbits) | gens.child_tag[g];
-}
-
-__device__ __forceinline__ uint32_t parent_skip(uint64_t v, const SortedLayout& L) {
-  // generator index + 1 to skip (0: none)
-  return L.prune ? static_cast<uint32_t>(v & L.tagmask) : 0u;
+  const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
+  return (keymix(child, L.km) << L.tagbits) | tag;
 }
 
 // Shared tile machinery.  Each thread holds up to PER tile elements in
@@ -161,19 +159,20 @@ __device__ __forceinline__ uint32_t reserve_run(uint32_t* __restrict__ cursor, u
   return off;
 }
 
-// TAGGED: every parent carries a tag (all layers after the start of an
-// inverse-closed set with pruning), so each yields exactly G-1 children and
-// twice as many parents fit a tile.
-template <int G, bool TAGGED>
+// MASKED: every parent carries a non-empty back-edge mask (all layers after
+// the start when back-edge pruning is on) and yields at most G-1 children,
+// so kTilePerThread / (G-1) parents fit a tile; each parent keeps G child
+// slots in registers, marked valid unless its generator is in the mask.
+template <int G, bool MASKED>
 struct ExpandTile {
-  static constexpr bool kCompact = TAGGED && G > 1;          // drop the skipped child by selects
-  static constexpr int kPerParent = kCompact ? G - 1 : G;
-  static constexpr int kParentsPerThread = (kTilePerThread / kPerParent) > 0 ? (kTilePerThread / kPerParent) : 1;
-  static constexpr int kRegs = kParentsPerThread * kPerParent;  // <= 32 (valid mask width)
+  static constexpr int kMaxKids = MASKED ? (G > 1 ? G - 1 : 1) : G;
+  static constexpr int kParentsPerThread = (kTilePerThread / kMaxKids) > 0 ? (kTilePerThread / kMaxKids) : 1;
+  static constexpr int kRegs = kParentsPerThread * G;  // <= 32 (valid mask width)
   static constexpr int kParents = kParentsPerThread * kRouteThreads;
+  static_assert(kParentsPerThread * kMaxKids <= kTilePerThread, "valid children fit the LDS tile");
 };
 
-template <int G, bool TAGGED>
+template <int G, bool MASKED>
 __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uint64_t* __restrict__ bank, uint64_t F,
                                                                         PackedGenerators gens, SortedLayout L,
                                                                         uint32_t K1, uint32_t NB, uint32_t P,
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
                                                                         uint32_t* __restrict__ cursors,
                                                                         uint32_t* __restrict__ error,
                                                                         uint64_t* __restrict__ out) {
-  using T = ExpandTile<G, TAGGED>;
+  using T = ExpandTile<G, MASKED>;
   static_assert(T::kRegs <= 32, "valid mask is 32 bits");
   __shared__ TileLds t;
   const uint32_t buckets = 1u << K1;
@@ -212,21 +211,17 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
       const uint64_t key = keyunmix(pv[m] >> L.tagbits, L.km);
-      const uint32_t skip = parent_skip(pv[m], L);
-      if constexpr (T::kCompact) {
-        // Every parent skips exactly generator skip-1; keep the other G-1.
-        uint64_t c[G];
+      uint32_t skip = MASKED ? static_cast<uint32_t>(pv[m] & L.tagmask) : 0u;
+      if (MASKED && live[m] && skip == 0) {
+        // A discovered state always has a back-edge; an empty mask would
+        // let G children into a tile sized for G-1 per parent.
+        atomicOr(error, kErrDomain);
+        skip = (1u << G) - 1u;
+      }
 #pragma unroll
-        for (int g = 0; g < G; ++g) c[g] = child_value(key, gens, g, L);
-#pragma unroll
-        for (int j = 0; j < G - 1; ++j) v[m * (G - 1) + j] = static_cast<uint32_t>(j) + 1 < skip ? c[j] : c[j + 1];
-        if (live[m]) valid |= ((1u << (G - 1)) - 1u) << (m * (G - 1));
-      } else {
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          v[m * G + g] = child_value(key, gens, g, L);
-          if (live[m] && static_cast<uint32_t>(g) + 1 != skip) valid |= 1u << (m * G + g);
-        }
+      for (int g = 0; g < G; ++g) {
+        v[m * G + g] = child_value(key, gens, g, L);
+        if (live[m] && !((skip >> g) & 1u)) valid |= 1u << (m * G + g);
       }
     }
     route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
@@ -291,17 +286,19 @@ struct UniqueArgs {
   uint32_t* fine_counts;
   uint32_t* error;
   unsigned long long* unique_total;
+  unsigned long long* cand_total;  // candidates seen (pruned expansions vary per parent)
   uint64_t tagmask;  // (1 << tagbits) - 1
-  uint64_t mark;     // tag value no candidate carries: "present in a bank"
+  uint64_t mark;     // top tag bit, never set in a candidate: "present in a bank"
+  uint32_t merge;    // OR the back-edge masks of duplicates into the kept entry
   uint32_t K, Kc, Kp, Ko, tagbits, buckets;
 };
 
 constexpr uint64_t kEmpty = ~0ull;
 
-// Bank entry h: if its state is in the table, overwrite that slot's tag
-// with `mark`.  Each slot matches at most one bank entry (the two banks are
-// disjoint layers of unique states), so a plain store suffices; concurrent
-// probes passing the slot see the same state bits either way.
+// Bank entry h: if its state is in the table, set `mark` in that slot's
+// tag.  Each slot matches at most one bank entry (a bank holds unique
+// states; two banks are disjoint layers), so a plain store suffices;
+// concurrent probes passing the slot see the same state bits either way.
 __device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, const UniqueArgs& a, uint32_t mask) {
   if (h == kEmpty) {
     atomicOr(&u.flags, 2u);
@@ -313,7 +310,7 @@ __device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, const Uniq
     const uint64_t v = u.table[slot];
     if (v == kEmpty) return;
     if ((v | a.tagmask) == key) {
-      u.table[slot] = (v & ~a.tagmask) | a.mark;
+      u.table[slot] = v | a.mark;
       return;
     }
     slot = (slot + 1) & mask;
@@ -400,7 +397,7 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     UniqueArgs a) {
   __shared__ UniqueLds u;
   const uint32_t fine_bits = a.Ko - a.K, fine = 1u << fine_bits;
-  unsigned long long unique_block = 0;
+  unsigned long long unique_block = 0, cand_block = 0;
   for (uint32_t f = blockIdx.x; f < a.buckets; f += gridDim.x) {
     const BucketBounds bcur = load_bounds(a, f);
     BucketData dcur;
@@ -445,7 +442,10 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
           claimed = true;
           break;
         }
-        if ((old | a.tagmask) == key) break;
+        if ((old | a.tagmask) == key) {
+          if (a.merge) atomicOr(reinterpret_cast<unsigned long long*>(&u.table[slot]), h & a.tagmask);
+          break;
+        }
         slot = (slot + 1) & mask;
       }
       if (claimed) {
@@ -461,13 +461,18 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // The all-ones state cannot live in the table (it is the empty marker).
     const bool max_survives = (u.flags & 3u) == 1u;
+    // Survivors are the claimed, unmarked slots; the table entry carries the
+    // merged back-edge mask of all duplicates.
     uint32_t survive = 0;
 #pragma unroll
-    for (int k = 0; k < kUniquePerThread; ++k)
-      if (slot_of[k] != kNoSlot && (u.table[slot_of[k]] & a.tagmask) != a.mark) {
-        survive |= 1u << k;
-        atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u);
-      }
+    for (int k = 0; k < kUniquePerThread; ++k) {
+      if (slot_of[k] == kNoSlot) continue;
+      const uint64_t kept = u.table[slot_of[k]];
+      if (kept & a.mark) continue;
+      dcur.v[k] = kept;
+      survive |= 1u << k;
+      atomicAdd(&u.fhist[mid_bits(kept, a.K, fine_bits)], 1u);
+    }
     if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
     __syncthreads();
     lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
@@ -481,10 +486,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     for (int k = 0; k < kUniquePerThread; ++k)
       if ((survive >> k) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u)] = dcur.v[k];
     if (threadIdx.x == 0 && max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
-    if (threadIdx.x == 0) unique_block += u.unique + (u.flags & 1u);
+    if (threadIdx.x == 0) {
+      unique_block += u.unique + (u.flags & 1u);
+      cand_block += c;
+    }
     __syncthreads();  // LDS reuse by the next bucket
   }
   if (threadIdx.x == 0 && unique_block) atomicAdd(a.unique_total, unique_block);
+  if (threadIdx.x == 0 && cand_block) atomicAdd(a.cand_total, cand_block);
 }
 
 // grid = 2^K blocks: survivors of sub-bucket f (at the start of its slab)
@@ -520,25 +529,25 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
   return hipGetLastError();
 }
 
-template <int G, bool TAGGED>
+template <int G, bool MASKED>
 static hipError_t expand_route_g(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
                                  uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                  uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                  hipStream_t stream) {
-  hipLaunchKernelGGL((expand_scatter_kernel<G, TAGGED>), dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L,
+  hipLaunchKernelGGL((expand_scatter_kernel<G, MASKED>), dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L,
                      K1, NB, P, cap, stride, err_bit, cursors, error, out);
   return hipGetLastError();
 }
 
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool all_tagged, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                hipStream_t stream) {
   if (K1 > kMaxPassBits || NB == 0 || P == 0 || cap == 0 || stride < cap) return hipErrorInvalidValue;
   switch (gens.count) {
 #define MGBFS_G(k)                                                                                               \
   case k:                                                                                                        \
-    return all_tagged                                                                                            \
+    return masked                                                                                                \
                ? expand_route_g<k, true>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
                                          stream)                                                            \
                : expand_route_g<k, false>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
@@ -563,17 +572,17 @@ hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t
 hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
                               const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
-                              unsigned long long* unique_total, uint32_t tagbits, uint32_t grid_limit,
-                              hipStream_t stream) {
+                              unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
+                              bool merge, uint32_t grid_limit, hipStream_t stream) {
   if (K > Ko || Ko - K > kMaxFineBits || (cur && K > Kc) || (prev && K > Kp) || grid_limit == 0)
     return hipErrorInvalidValue;
   if (tagbits < 4 || tagbits > 60) return hipErrorInvalidValue;
   const uint64_t tagmask = (1ull << tagbits) - 1;
-  // Tags in use are 0 and 1..G (G + 1 < tagmask when tags are used), so
-  // tagmask - 1 is never a candidate's tag and never turns a state into
-  // the all-ones empty marker.
-  UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, tagmask, tagmask - 1,
-               K, Kc, Kp, Ko, tagbits, 1u << K};
+  // Candidate tags use at most bits 0 .. tagbits-3 (back-edge masks need
+  // G <= tagbits - 2), so the mark bit tagbits-1 is free and a marked entry
+  // never equals the all-ones empty marker.
+  UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, cand_total, tagmask,
+               1ull << (tagbits - 1), merge ? 1u : 0u, K, Kc, Kp, Ko, tagbits, 1u << K};
   const uint32_t grid = std::min<uint32_t>(1u << K, grid_limit);
   hipLaunchKernelGGL(unique_diff_kernel, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
   return hipGetLastError();

@@ -87,6 +87,49 @@ def test_s11_sorted_overflow_retry_path_is_exact():
     assert info["retries"] > 0
 
 
+def cross_layer_edges(n, layers_keys):
+    """Edges between consecutive BFS layers of LRX on S_n, counted by brute
+    force over the oracle's layer sets (each counted once, from the deeper end)."""
+    gens = O.lrx_generators(n)
+    sets = [set(k) for k in layers_keys]
+    total = 0
+    for d in range(1, len(sets)):
+        for key in sets[d]:
+            s = O.unpack(key, n)
+            total += sum(O.pack([s[g[i]] for i in range(n)]) in sets[d - 1] for g in gens)
+    return total
+
+
+@pytest.mark.parametrize("n", [8, 9, 10])
+def test_sorted_back_edge_masks_skip_every_back_edge(n):
+    # With back-edge masks each state skips all generators leading to the
+    # previous layer: candidates = 3 n! - (edges between consecutive layers).
+    # For even n, L, R and X are odd permutations, so the graph is bipartite
+    # by parity and all 3 n! / 2 edges join consecutive layers.
+    ref = O.packed_bfs(n, keep_keys=n < 10)
+    cross = 3 * math.factorial(n) // 2 if n == 10 else cross_layer_edges(n, ref["keys"])
+    if n == 8:
+        assert cross == 3 * math.factorial(n) // 2
+    with NativeEngine(lrx(n), backend="sorted") as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref["layers"]
+    assert info["history_layers"] == 2
+    assert info["candidates"] == 3 * math.factorial(n) - cross
+
+
+@pytest.mark.parametrize("n", [9, 10])
+def test_sorted_without_pruning_is_exact(n):
+    # Flag 8: no back-edge masks, every child generated, both banks probed.
+    ref = O.packed_bfs(n)["layers"]
+    with NativeEngine(lrx(n), backend="sorted", flags=8) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref
+    assert info["history_layers"] == 3
+    assert info["candidates"] == 3 * math.factorial(n)
+
+
 def test_s11_sorted_lane_slab_retry_path_is_exact():
     # Owner-lane slabs sized to the mean overflow; the layer is redone with slack.
     ref = O.packed_bfs(11)["layers"]
