@@ -16,8 +16,8 @@ constexpr uint32_t kMaxPassBits = 10;     // buckets per routing pass (LDS count
 constexpr uint32_t kMaxBucketBits = 20;   // sub-buckets per layer: 2^20
 constexpr uint32_t kMaxFineBits = 10;     // bank partition bits beyond the layer's K
 constexpr uint32_t kUniqueSlots = 8192;   // LDS hash slots per sub-bucket
-constexpr uint32_t kUniqueCap = 4096;     // candidates per sub-bucket (load factor <= 0.5)
-constexpr uint32_t kSlabStride = 4096;    // entries between sub-bucket slab starts (>= kUniqueCap)
+constexpr uint32_t kUniqueCap = 5120;     // candidates per sub-bucket (table load factor <= 0.625)
+constexpr uint32_t kSlabStride = 5120;    // entries between sub-bucket slab starts (>= kUniqueCap)
 
 // Stored-value layout of the sorted backend: (keymix(key) << tagbits) | tag.
 struct SortedLayout {

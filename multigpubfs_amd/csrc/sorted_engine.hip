@@ -28,7 +28,10 @@ uint32_t ceil_log2(uint64_t x) {
   return b;
 }
 
-constexpr uint64_t kTargetBucket = 3072;  // average candidates per sub-bucket (cap 4096)
+// Average candidates per sub-bucket: large sub-buckets amortise each
+// workgroup's fixed work (bounds, table clear, fine-count scan); the cap is
+// kUniqueCap = 5120 and counts stay within a few percent of the mean.
+constexpr uint64_t kTargetBucket = 4800;
 constexpr uint32_t kPhaseCount = 4;       // expand, partition, unique, compact
 constexpr uint32_t kLaneParts = 32;       // slabs per owner lane (cursor contention / P)
 
@@ -104,6 +107,7 @@ class SortedEngine final : public Engine {
     cur_ = 0;
     prev_ = -1;
     slack_ = (flags_ & MGBFS_FLAG_TEST_TIGHT_SLABS) ? 0 : 1;
+    last_ratio_ = 0.0;
     for (int b = 0; b < 3; ++b) count_[b] = 0;
     count_[0] = 1;
     ko_[0] = kMaxBucketBits;
@@ -139,7 +143,12 @@ class SortedEngine final : public Engine {
     // back-edge mask (every state but the start does); the slabs and the
     // bucket bits are sized for that bound.
     const bool masked = layout_.prune && layers.size() > 1;
-    const uint64_t Cest = masked ? F * (G > 1 ? G - 1 : 1) : Cmax;
+    uint64_t Cest = masked ? F * (G > 1 ? G - 1 : 1) : Cmax;
+    // Masked layers skip every back-edge, so the previous layer's measured
+    // children per parent (plus 10%) predicts this one closely; a low guess
+    // costs one retry of the layer (slab or sub-bucket overflow).
+    if (masked && last_ratio_ > 0)
+      Cest = std::min<uint64_t>(Cest, static_cast<uint64_t>(static_cast<double>(F) * last_ratio_ * 1.1) + 4096);
     uint32_t K = Cest <= kTargetBucket ? 0 : ceil_log2((Cest + kTargetBucket - 1) / kTargetBucket);
     // Test hook: start three bits coarse so large layers exercise the
     // bucket-overflow retry path.
@@ -219,6 +228,7 @@ class SortedEngine final : public Engine {
         return MGBFS_RESOURCE;
       }
       candidates += C;
+      last_ratio_ = masked ? static_cast<double>(C) / static_cast<double>(F) : 0.0;
       unique_candidates += host_[0];
       if (N == 0) {
         *next_count = 0;
@@ -362,6 +372,7 @@ class SortedEngine final : public Engine {
   PackedGenerators gens_;
   SortedLayout layout_{};
   uint32_t n_, margin_ = 0, flags_ = 0, unique_grid_ = 2048, slack_ = 1;
+  double last_ratio_ = 0.0;  // children per parent of the last masked layer
   uint64_t start_key_ = 0, budget_ = 0, cap_limit_ = 0, lane_cap_ = 0, sub_cap_ = 0;
   DeviceBuffer bank_[3], off_[3], lane_, sub_, lcount_, scount_, scan_ws_, fine_counts_, control_;
   uint64_t bank_cap_[3] = {0, 0, 0}, count_[3] = {0, 0, 0};

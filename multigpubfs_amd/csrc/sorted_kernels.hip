@@ -410,9 +410,10 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = 0;
       continue;
     }
-    // Table sized to the bucket: slots = next_pow2(2c) (load factor <= 0.5).
+    // Table sized to the bucket: slots = next_pow2(2c) (load factor <= 0.5),
+    // at most kUniqueSlots (load factor <= kUniqueCap / kUniqueSlots).
     uint32_t slots = 64;
-    while (slots < 2 * c) slots <<= 1;
+    while (slots < 2 * c && slots < kUniqueSlots) slots <<= 1;
     const uint32_t mask = slots - 1;
     ulonglong2* table2 = reinterpret_cast<ulonglong2*>(u.table);
     for (uint32_t i = threadIdx.x; i < slots / 2; i += kUniqueThreads) table2[i] = make_ulonglong2(kEmpty, kEmpty);
