@@ -81,9 +81,9 @@ class SortedEngine final : public Engine {
     history_layers = layout_.prune ? 2 : 3;
     margin_ = std::min<uint32_t>(kMaxFineBits, 2 * (ceil_log2(gens_.count) + 1));
     start_key_ = g.start_key();
-    // One workgroup per sub-bucket measured faster than a persistent grid
-    // with next-bucket prefetch (the kernel is bound by LDS atomics, not by
-    // load latency), so the grid is not capped.
+    // One workgroup per sub-bucket: persistent grids of 2 or 8 workgroups
+    // per CU, prefetching the next sub-bucket's bounds, measured slower
+    // (127 and 122 vs 120 ms per S13 BFS).
     unique_grid_ = 1u << kMaxBucketBits;
     control_.allocate(64);
     fine_counts_.allocate(((1ull << kMaxBucketBits) + 1) * 4);
