@@ -20,10 +20,12 @@
 #include <csignal>
 #include <cstdio>
 #include <fstream>
+#include <map>
 #include <iostream>
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "../../include/mgbfs_hip.h"
@@ -262,10 +264,36 @@ Json rows_json(const std::vector<int64_t>& flat, uint64_t count, uint32_t width)
   return arr;
 }
 
-int cmd_graph(const std::vector<std::string>& args) {
+// An engine kept alive between resident-session jobs: a job whose graph,
+// device and options match the previous one resets it instead of
+// allocating a new one.
+struct EngineCache {
+  std::string key;
+  void* engine = nullptr;
+  uint64_t device_bytes = 0;
+  void release() {
+    if (engine) mgbfs_engine_destroy(engine);
+    engine = nullptr;
+    key.clear();
+    device_bytes = 0;
+  }
+  ~EngineCache() { release(); }
+};
+
+bool path_exists(const std::string& path) {
+  struct stat st;
+  return !path.empty() && stat(path.c_str(), &st) == 0;
+}
+
+// One `graph` job: writes <out>/states.json and <out>/report.json and returns
+// the report.  default_device applies when no --device is given; with a
+// cache the engine survives the job; cancel_path (if it appears between
+// layers) ends the run INCOMPLETE / CANCELLED like SIGTERM.
+Json run_graph_job(const std::vector<std::string>& args, int32_t default_device, EngineCache* cache,
+                   const std::string& cancel_path, bool* reused) {
   if (args.size() < 2)
     throw CliError("CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N]");
-  int32_t device = 0;
+  int32_t device = default_device;
   uint64_t capacity = 0, seconds = 0;
   std::string backend = "auto";
   for (size_t i = 2; i < args.size(); i += 2) {
@@ -277,12 +305,12 @@ int cmd_graph(const std::vector<std::string>& args) {
     else if (k == "--backend") backend = args[i + 1];
     else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
   }
+  if (cache && device != default_device) throw CliError("GENERIC_SESSION_DEVICE");
   const char* world = std::getenv("WORLD_SIZE");
   if (world && std::string(world) != "1") throw CliError("GENERIC_DISTRIBUTED_INTEGRATION_NOT_READY");
   GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
   const std::string out = args[1];
-  struct stat st;
-  if (stat(out.c_str(), &st) == 0) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
+  if (path_exists(out)) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
   if (mkdir(out.c_str(), 0755)) throw CliError("CLI_GRAPH_OUTPUT_CREATE");
 
   std::vector<uint32_t> pflat;
@@ -292,10 +320,26 @@ int cmd_graph(const std::vector<std::string>& args) {
   opts.device = device;
   opts.backend = backend_code(backend);
   opts.capacity = capacity;
+  const std::string key = graph.digest() + "|" + std::to_string(device) + "|" + backend + "|" + std::to_string(capacity);
   void* engine = nullptr;
   char err[256] = {0};
   const auto setup0 = std::chrono::steady_clock::now();
-  if (mgbfs_engine_create(&desc, &opts, &engine, err, sizeof err)) throw CliError(err[0] ? err : "ENGINE_CREATE");
+  if (reused) *reused = false;
+  if (cache && cache->engine && cache->key == key) {
+    if (mgbfs_engine_reset(cache->engine)) {
+      cache->release();
+      throw CliError("ENGINE_RESET");
+    }
+    engine = cache->engine;
+    if (reused) *reused = true;
+  } else {
+    if (cache) cache->release();
+    if (mgbfs_engine_create(&desc, &opts, &engine, err, sizeof err)) throw CliError(err[0] ? err : "ENGINE_CREATE");
+    if (cache) {
+      cache->engine = engine;
+      cache->key = key;
+    }
+  }
   const double setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - setup0).count();
   std::signal(SIGTERM, on_signal);
   std::signal(SIGINT, on_signal);
@@ -304,7 +348,7 @@ int cmd_graph(const std::vector<std::string>& args) {
   std::string status = "COMPLETE", reason = "FRONTIER_EXHAUSTED";
   const auto t0 = std::chrono::steady_clock::now();
   while (true) {
-    if (g_cancel) {
+    if (g_cancel || path_exists(cancel_path)) {
       status = "INCOMPLETE";
       reason = "CANCELLED";
       break;
@@ -319,7 +363,8 @@ int cmd_graph(const std::vector<std::string>& args) {
     uint32_t fatal = 0;
     const int s = mgbfs_engine_advance(engine, &next, &fatal);
     if (s < 0) {
-      mgbfs_engine_destroy(engine);
+      if (cache) cache->release();
+      else mgbfs_engine_destroy(engine);
       throw CliError(std::string("ENGINE_ADVANCE: ") + mgbfs_engine_backend_name(nullptr));
     }
     if (s == MGBFS_LAYER) {
@@ -389,11 +434,168 @@ int cmd_graph(const std::vector<std::string>& args) {
   report["kernel_seconds"] = Json::real(info.kernel_seconds);
   report["retention"] = Json::str("current sample <=1000 and previous layer only if <1000");
   report["scope"] = Json::str("single-device MI355X engine");
-  mgbfs_engine_destroy(engine);
+  if (cache) cache->device_bytes = info.device_bytes;
+  else mgbfs_engine_destroy(engine);
   write_file(out + "/states.json", states_text);
   write_file(out + "/report.json", mgbfs::host::dumps(report));
+  return report;
+}
+
+int cmd_graph(const std::vector<std::string>& args) {
+  const Json report = run_graph_job(args, 0, nullptr, "", nullptr);
   std::cout << mgbfs::host::dumps(report) << std::endl;
   return 0;
+}
+
+// ------------------------------------------------------------ resident session
+
+std::string seq_name(const std::string& root, const char* stem, uint64_t sequence, const char* ext) {
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%08llu", static_cast<unsigned long long>(sequence));
+  return root + "/" + stem + "-" + buf + ext;
+}
+
+void write_atomically(const std::string& path, const std::string& text) {
+  const std::string tmp = path + ".tmp";
+  write_file(tmp, text);
+  if (std::rename(tmp.c_str(), path.c_str())) throw CliError("GENERIC_SESSION_WRITE");
+}
+
+void set_phase(const std::string& path, uint64_t sequence, const char* phase) {
+  if (path.empty()) return;
+  Json v = Json::object();
+  v["schema"] = Json::integer(1);
+  v["pid"] = Json::integer(static_cast<int64_t>(getpid()));
+  v["sequence"] = Json::str(std::to_string(sequence));
+  v["phase"] = Json::str(phase);
+  write_atomically(path, mgbfs::host::dumps(v));
+}
+
+Json inventory_json(int32_t device, const EngineCache& cache) {
+  uint32_t cus = 0, wave = 0, lds = 0;
+  uint64_t total = 0, free_bytes = 0;
+  if (mgbfs_hip_device_info(device, &cus, &wave, &lds, &total, &free_bytes)) throw CliError("GENERIC_SESSION_INVENTORY");
+  Json v = Json::object();
+  v["device"] = Json::integer(device);
+  v["free_bytes"] = Json::unsigned_(std::min(total, free_bytes + cache.device_bytes));
+  v["total_bytes"] = Json::unsigned_(total);
+  v["cached_reusable_bytes"] = Json::unsigned_(cache.device_bytes);
+  v["additional_reserve_bytes"] = Json::unsigned_(2ull << 30);
+  v["sm_count"] = Json::integer(cus);
+  v["warp_size"] = Json::integer(wave);
+  v["shared_bytes_per_block"] = Json::integer(lds);
+  return v;
+}
+
+// Resident worker (multigpubfs/generic_session.py protocol, schema 1): jobs
+// arrive as <root>/job-NNNNNNNN.json {schema, sequence, command, args, env};
+// each gets <root>/response-NNNNNNNN.json {schema, sequence, ok, result |
+// error, cache}.  A `shutdown` file ends the loop.  Commands: graph (the
+// `mgbfs graph` arguments; the device is the worker's), inventory,
+// release-memory.  Job env keys must start with MGBFS_ or NCCL_ and are
+// undone before the next job.  After a failed job the worker exits: a rank
+// is never reused after an error.
+int cmd_graph_session(const std::vector<std::string>& args) {
+  if (args.size() != 3) throw CliError("CLI_USAGE: mgbfs graph-session <root> <rank> <device>");
+  const std::string root = args[0];
+  const uint64_t rank = parse_u64(args[1], "GENERIC_SESSION_RANK");
+  const int32_t device = static_cast<int32_t>(parse_u64(args[2], "GENERIC_SESSION_DEVICE"));
+  (void)rank;
+  EngineCache cache;
+  std::map<std::string, std::pair<bool, std::string>> previous;  // name -> (was set, old value)
+  std::signal(SIGTERM, on_signal);
+  std::signal(SIGINT, on_signal);
+  for (uint64_t sequence = 0;; ++sequence) {
+    const std::string job_path = seq_name(root, "job", sequence, ".json");
+    const std::string shutdown = root + "/shutdown";
+    uint32_t nap_us = 200;
+    while (!path_exists(job_path)) {
+      if (path_exists(shutdown) || g_cancel) return 0;
+      usleep(nap_us);
+      nap_us = std::min<uint32_t>(nap_us * 2, 20000);
+    }
+    bool ok = true;
+    std::string error;
+    Json result = Json::object();
+    std::string phase_path;
+    try {
+      const Json job = mgbfs::host::JsonParser(read_file(job_path)).parse();
+      if (!job.has("schema") || !job.at("schema").is_int() || job.at("schema").as_int() != 1 ||
+          !job.has("sequence") || !job.at("sequence").is_int() ||
+          job.at("sequence").as_int() != static_cast<int64_t>(sequence))
+        throw CliError("GENERIC_SESSION_SEQUENCE");
+      for (const auto& kv : previous) {
+        if (kv.second.first) setenv(kv.first.c_str(), kv.second.second.c_str(), 1);
+        else unsetenv(kv.first.c_str());
+      }
+      previous.clear();
+      if (job.has("env")) {
+        const Json& env = job.at("env");
+        if (env.kind != Json::Obj) throw CliError("GENERIC_SESSION_ENV");
+        for (const auto& kv : env.o)
+          if (kv.second.kind != Json::Str || (kv.first.rfind("MGBFS_", 0) != 0 && kv.first.rfind("NCCL_", 0) != 0))
+            throw CliError("GENERIC_SESSION_ENV");
+        for (const auto& kv : env.o) {
+          const char* old = std::getenv(kv.first.c_str());
+          previous[kv.first] = {old != nullptr, old ? old : ""};
+          setenv(kv.first.c_str(), kv.second.s.c_str(), 1);
+        }
+      }
+      if (const char* p = std::getenv("MGBFS_JOB_PHASE_PATH")) phase_path = p;
+      set_phase(phase_path, sequence, "running");
+      const std::string command = job.has("command") && job.at("command").kind == Json::Str ? job.at("command").s : "";
+      std::vector<std::string> cargs;
+      if (job.has("args")) {
+        if (job.at("args").kind != Json::Arr) throw CliError("GENERIC_SESSION_ARGS");
+        for (const Json& a : job.at("args").a) {
+          if (a.kind != Json::Str) throw CliError("GENERIC_SESSION_ARGS");
+          cargs.push_back(a.s);
+        }
+      }
+      if (command == "graph") {
+        bool reused = false;
+        const Json report = run_graph_job(cargs, device, &cache, seq_name(root, "cancel", sequence, ""), &reused);
+        result["pid"] = Json::integer(static_cast<int64_t>(getpid()));
+        result["status"] = report.at("status");
+        result["reason"] = report.at("reason");
+        result["output"] = Json::str(cargs.size() > 1 ? cargs[1] : "");
+        result["engine_reused"] = Json::boolean(reused);
+      } else if (command == "inventory") {
+        result = inventory_json(device, cache);
+      } else if (command == "release-memory") {
+        cache.release();
+        result["released"] = Json::boolean(true);
+      } else {
+        throw CliError("GENERIC_SESSION_COMMAND");
+      }
+    } catch (const std::exception& e) {
+      ok = false;
+      error = e.what();
+    }
+    Json response = Json::object();
+    response["schema"] = Json::integer(1);
+    response["sequence"] = Json::unsigned_(sequence);
+    response["ok"] = Json::boolean(ok);
+    if (ok) response["result"] = result;
+    else response["error"] = Json::str(error);
+    Json cache_stats = Json::object();
+    cache_stats["cached_engines"] = Json::integer(cache.engine ? 1 : 0);
+    cache_stats["cached_bytes"] = Json::unsigned_(cache.device_bytes);
+    response["cache"] = cache_stats;
+    try {
+      set_phase(phase_path, sequence, ok ? "done" : "failed");
+    } catch (const std::exception&) {
+    }
+    write_atomically(seq_name(root, "response", sequence, ".json"), mgbfs::host::dumps(response));
+    if (!ok) {
+      cache.release();
+      Json err = Json::object();
+      err["status"] = Json::str("ERROR");
+      err["error"] = Json::str(error);
+      std::cerr << mgbfs::host::dumps(err) << std::endl;
+      return 1;
+    }
+  }
 }
 
 int cmd_graph_count() {
@@ -486,10 +688,12 @@ int cmd_key_info() {
 int main(int argc, char** argv) {
   std::vector<std::string> args(argv + 1, argv + argc);
   try {
-    if (args.empty()) throw CliError("CLI_USAGE: mgbfs <graph|graph-info|graph-count|key-info|version> ...");
+    if (args.empty())
+      throw CliError("CLI_USAGE: mgbfs <graph|graph-session|graph-info|graph-count|graph-digest|key-info|version> ...");
     const std::string cmd = args[0];
     std::vector<std::string> rest(args.begin() + 1, args.end());
     if (cmd == "graph") return cmd_graph(rest);
+    if (cmd == "graph-session") return cmd_graph_session(rest);
     if (cmd == "graph-info") return cmd_graph_info(rest);
     if (cmd == "graph-count") return cmd_graph_count();
     if (cmd == "graph-digest") return cmd_graph_digest(rest);

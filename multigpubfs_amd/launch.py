@@ -98,7 +98,15 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
             cmd += ["--capacity", str(capacity)]
         if max_seconds:
             cmd += ["--seconds", str(max_seconds)]
-        proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
-    if proc.returncode:
-        raise NativeError(_error_code(proc.stderr), proc.stderr)
+        from . import session
+        pool = session.active_pool()
+        if pool is not None:
+            # Resident worker: same arguments, one job file instead of a process.
+            job = pool.worker_for(exe, device).submit("graph", cmd[2:])
+            if job.wait(timeout):
+                raise NativeError(job.error or "GENERIC_SESSION_JOB_FAILED", json.dumps(job.response))
+        else:
+            proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
+            if proc.returncode:
+                raise NativeError(_error_code(proc.stderr), proc.stderr)
     return json.loads((output / "report.json").read_text())
