@@ -269,9 +269,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
 
 struct UniqueLds {
   alignas(16) uint64_t table[kUniqueSlots];  // read two slots at a time
-  uint32_t fhist[1u << kMaxFineBits];
-  uint32_t fstart[1u << kMaxFineBits];
-  uint32_t wave_sums[kUniqueThreads / kWave];
+  uint32_t fhist[1u << kMaxFineBits];        // survivors, then write cursors, per fine bucket
   uint32_t flags;  // bit0: candidate ~0 present, bit1: ~0 found in a bank
   uint32_t unique;
 };
@@ -317,21 +315,12 @@ __device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, const Uniq
   }
 }
 
-__device__ __forceinline__ void prefetch_slice(const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
-                                               uint64_t (&pre)[kPrefetch]) {
-#pragma unroll
-  for (int k = 0; k < kPrefetch; ++k) {
-    const uint64_t i = lo + threadIdx.x + static_cast<uint64_t>(k) * kUniqueThreads;
-    pre[k] = i < hi ? bank[i] : 0;
-  }
-}
-
-__device__ __forceinline__ void probe_bank(UniqueLds& u, const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
-                                           const uint64_t (&pre)[kPrefetch], const UniqueArgs& a, uint32_t mask) {
-#pragma unroll
-  for (int k = 0; k < kPrefetch; ++k)
-    if (lo + threadIdx.x + static_cast<uint64_t>(k) * kUniqueThreads < hi) probe_value(u, pre[k], a, mask);
-  for (uint64_t base = lo + threadIdx.x + kPrefetch * kUniqueThreads; base < hi; base += kPrefetch * kUniqueThreads) {
+// Bank entries [lo + skip, hi), skip = entries per thread already probed
+// from the prefetch, in chunks of kPrefetch loads per thread.
+__device__ __forceinline__ void probe_rest(UniqueLds& u, const uint64_t* __restrict__ bank, uint64_t lo, uint64_t hi,
+                                           uint32_t skip, const UniqueArgs& a, uint32_t mask) {
+  for (uint64_t base = lo + threadIdx.x + static_cast<uint64_t>(skip) * kUniqueThreads; base < hi;
+       base += kPrefetch * kUniqueThreads) {
     uint64_t v[kPrefetch];
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
@@ -368,10 +357,26 @@ __device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_
   return b;
 }
 
+// Candidates and the first bank entries, loaded together: 2 kPrefetch
+// entries per thread, all from the current bank when there is no previous
+// one (back-edge masks), else kPrefetch from each.  Slot k holds entry
+// slice_lo + threadIdx.x + j * kUniqueThreads of its bank.
+constexpr int kBankPrefetch = 2 * kPrefetch;
+
 struct BucketData {
   uint64_t v[kUniquePerThread];
-  uint64_t cpre[kPrefetch], ppre[kPrefetch];
+  uint64_t bpre[kBankPrefetch];
 };
+
+__device__ __forceinline__ int cur_prefetch(const UniqueArgs& a) { return a.prev ? kPrefetch : kBankPrefetch; }
+
+__device__ __forceinline__ bool bank_slot(const UniqueArgs& a, const BucketBounds& b, int k, uint64_t* index) {
+  const int ncur = cur_prefetch(a);
+  const bool cur = k < ncur;
+  const uint64_t i = (cur ? b.clo : b.plo) + threadIdx.x + static_cast<uint64_t>(cur ? k : k - ncur) * kUniqueThreads;
+  *index = i;
+  return i < (cur ? b.chi : b.phi);
+}
 
 __device__ __forceinline__ void load_data(const UniqueArgs& a, const BucketBounds& b, BucketData& d) {
   const bool fits = b.e - b.s <= kUniqueCap;
@@ -380,8 +385,13 @@ __device__ __forceinline__ void load_data(const UniqueArgs& a, const BucketBound
     const uint64_t i = b.s + k * kUniqueThreads + threadIdx.x;
     d.v[k] = (fits && i < b.e) ? a.cand[i] : 0;
   }
-  prefetch_slice(a.cur, b.clo, b.chi, d.cpre);
-  prefetch_slice(a.prev, b.plo, b.phi, d.ppre);
+  const int ncur = cur_prefetch(a);
+#pragma unroll
+  for (int k = 0; k < kBankPrefetch; ++k) {
+    uint64_t i;
+    const bool live = bank_slot(a, b, k, &i);
+    d.bpre[k] = live ? (k < ncur ? a.cur : a.prev)[i] : 0;
+  }
 }
 
 constexpr uint32_t kNoSlot = ~0u;
@@ -457,8 +467,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     if (inserted) atomicAdd(&u.unique, inserted);
     __syncthreads();
     // Difference against the two retained layers (same sub-bucket only).
-    probe_bank(u, a.cur, bcur.clo, bcur.chi, dcur.cpre, a, mask);
-    probe_bank(u, a.prev, bcur.plo, bcur.phi, dcur.ppre, a, mask);
+#pragma unroll
+    for (int k = 0; k < kBankPrefetch; ++k) {
+      uint64_t i;
+      if (bank_slot(a, bcur, k, &i)) probe_value(u, dcur.bpre[k], a, mask);
+    }
+    const int ncur = cur_prefetch(a);
+    probe_rest(u, a.cur, bcur.clo, bcur.chi, ncur, a, mask);
+    if (a.prev) probe_rest(u, a.prev, bcur.plo, bcur.phi, kBankPrefetch - ncur, a, mask);
     __syncthreads();
     // The all-ones state cannot live in the table (it is the empty marker).
     const bool max_survives = (u.flags & 3u) == 1u;
@@ -476,10 +492,22 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     }
     if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
     __syncthreads();
-    lds_exclusive_scan(u.fhist, u.fstart, fine, u.wave_sums);
-    for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) {
-      a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + i] = u.fhist[i];
-      u.fhist[i] = u.fstart[i];
+    // One wave turns the fine counts into write cursors (exclusive scan by
+    // shuffles, <= 16 counters per lane) while the others wait at a single
+    // barrier.
+    if (threadIdx.x < kWave) {
+      const uint32_t per = (fine + kWave - 1) / kWave, base = threadIdx.x * per;
+      uint32_t sum = 0;
+      for (uint32_t j = 0; j < per; ++j)
+        if (base + j < fine) sum += u.fhist[base + j];
+      uint32_t run = wave_incl_scan(sum) - sum;
+      for (uint32_t j = 0; j < per; ++j)
+        if (base + j < fine) {
+          const uint32_t c = u.fhist[base + j];
+          a.fine_counts[(static_cast<uint64_t>(f) << fine_bits) + base + j] = c;
+          u.fhist[base + j] = run;
+          run += c;
+        }
     }
     __syncthreads();
     // Every survivor is in registers: write it back in place.
