@@ -357,10 +357,10 @@ __device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_
   return b;
 }
 
-// Candidates and the first bank entries, loaded together: 2 kPrefetch
-// entries per thread, all from the current bank when there is no previous
-// one (back-edge masks), else kPrefetch from each.  Slot k holds entry
-// slice_lo + threadIdx.x + j * kUniqueThreads of its bank.
+// The first bank entries: 2 kPrefetch per thread, all from the current
+// bank when there is no previous one (back-edge masks), else kPrefetch from
+// each.  Slot k holds entry slice_lo + threadIdx.x + j * kUniqueThreads of
+// its bank.
 constexpr int kBankPrefetch = 2 * kPrefetch;
 
 struct BucketData {
@@ -378,13 +378,17 @@ __device__ __forceinline__ bool bank_slot(const UniqueArgs& a, const BucketBound
   return i < (cur ? b.chi : b.phi);
 }
 
-__device__ __forceinline__ void load_data(const UniqueArgs& a, const BucketBounds& b, BucketData& d) {
-  const bool fits = b.e - b.s <= kUniqueCap;
+// All kUniqueCap slots of slab f: the slab start is known without the
+// bounds, so these loads go out beside the bounds loads and one dependent
+// round trip leaves the critical path.  Slots past the count hold stale
+// values and are ignored.
+__device__ __forceinline__ void load_slab(const UniqueArgs& a, uint32_t f, BucketData& d) {
+  const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
 #pragma unroll
-  for (int k = 0; k < kUniquePerThread; ++k) {
-    const uint64_t i = b.s + k * kUniqueThreads + threadIdx.x;
-    d.v[k] = (fits && i < b.e) ? a.cand[i] : 0;
-  }
+  for (int k = 0; k < kUniquePerThread; ++k) d.v[k] = a.cand[s + k * kUniqueThreads + threadIdx.x];
+}
+
+__device__ __forceinline__ void load_bank(const UniqueArgs& a, const BucketBounds& b, BucketData& d) {
   const int ncur = cur_prefetch(a);
 #pragma unroll
   for (int k = 0; k < kBankPrefetch; ++k) {
@@ -409,9 +413,10 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
   const uint32_t fine_bits = a.Ko - a.K, fine = 1u << fine_bits;
   unsigned long long unique_block = 0, cand_block = 0;
   for (uint32_t f = blockIdx.x; f < a.buckets; f += gridDim.x) {
-    const BucketBounds bcur = load_bounds(a, f);
     BucketData dcur;
-    load_data(a, bcur, dcur);
+    load_slab(a, f, dcur);  // in flight together with the bounds loads
+    const BucketBounds bcur = load_bounds(a, f);
+    load_bank(a, bcur, dcur);
     const uint64_t s = bcur.s, e = bcur.e;
     const uint64_t c = e - s;
     if (c > kUniqueCap) {
