@@ -57,6 +57,8 @@ extern "C" {
                                              no slack, so lane overflows take the retry path */
 #define MGBFS_FLAG_TEST_NO_PRUNE 8u       /* sorted backend: no back-edge masks; every
                                              child is generated and both banks are probed */
+#define MGBFS_FLAG_MAX_CAPACITY 16u       /* keep 256 MiB of free VRAM in reserve instead of
+                                             2 GiB (reference memory_preset "max_capacity") */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are

@@ -120,7 +120,8 @@ def _search(original, graph, backend, options, output, kwargs):
     output = Path(output).absolute()
     devices = list(options.devices) if options.devices is not None else None
     report = run_graph(definition, output, devices=devices, max_seconds=options.max_seconds,
-                       autotune=options.autotune, memory_backend=options.memory_backend)
+                       autotune=options.autotune, memory_backend=options.memory_backend,
+                       segment_rows=options.segment_rows, memory_preset=options.memory_preset)
     snapshot = json.loads((output / "states.json").read_text())
     sizes = report["layer_sizes"]
     samples, layers = {}, {}

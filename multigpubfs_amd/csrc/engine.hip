@@ -80,11 +80,14 @@ void Engine::fill_info(mgbfs_engine_info& info) const {
   info.retries = retry_count();
 }
 
-static uint64_t vram_budget(const mgbfs_engine_options& o) {
+// Device bytes an engine may use: the caller's budget, else free memory
+// minus a reserve of 2 GiB, or 256 MiB under MGBFS_FLAG_MAX_CAPACITY (the
+// cold and warm reserves of generic_memory.rs:9-19).
+uint64_t vram_budget(const mgbfs_engine_options& o) {
   if (o.vram_budget) return o.vram_budget;
   size_t fr = 0, tot = 0;
   MGBFS_HIP(hipMemGetInfo(&fr, &tot));
-  const uint64_t reserve = 2ull << 30;  // generic_memory.rs:9-19 cold reserve
+  const uint64_t reserve = (o.flags & MGBFS_FLAG_MAX_CAPACITY) ? (256ull << 20) : (2ull << 30);
   return fr > reserve ? fr - reserve : 0;
 }
 

@@ -106,6 +106,7 @@ Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o);
 Engine* make_generic_engine(const Graph& g, const mgbfs_engine_options& o, uint64_t state_bound);
 void set_last_error(const char* what);
 Graph intake_graph(const mgbfs_graph_desc* d);
+uint64_t vram_budget(const mgbfs_engine_options& o);
 // Mathematical bound on reachable states (0 = unknown), capped at 2^63.
 uint64_t state_space_bound(const Graph& g);
 // Index of each generator's inverse in the set (-1 when absent).

@@ -66,12 +66,7 @@ class GenericEngine final : public Engine {
     uint64_t cap = o.capacity;
     if (!cap) {
       // Largest capacity whose plan fits the budget (generic_native.rs:59-64).
-      uint64_t budget = o.vram_budget;
-      if (!budget) {
-        size_t fr = 0, tot = 0;
-        MGBFS_HIP(hipMemGetInfo(&fr, &tot));
-        budget = fr > (2ull << 30) ? fr - (2ull << 30) : 0;
-      }
+      const uint64_t budget = vram_budget(o);
       uint64_t lo = 1, hi = std::min<uint64_t>(kMaxArena, state_bound ? state_bound : kMaxArena);
       if (generic_plan_bytes(elements_, 1, state_bytes, nullptr) > budget) throw std::runtime_error("GENERIC_VRAM_CAPACITY");
       while (lo < hi) {

@@ -296,6 +296,7 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
   int32_t device = default_device;
   uint64_t capacity = 0, seconds = 0;
   std::string backend = "auto";
+  uint32_t flags = 0;
   for (size_t i = 2; i < args.size(); i += 2) {
     if (i + 1 >= args.size()) throw CliError("CLI_GRAPH_OPTION_VALUE");
     const std::string& k = args[i];
@@ -303,7 +304,10 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     else if (k == "--capacity") capacity = parse_u64(args[i + 1], "CLI_GRAPH_CAPACITY");
     else if (k == "--seconds") seconds = parse_u64(args[i + 1], "CLI_GRAPH_SECONDS");
     else if (k == "--backend") backend = args[i + 1];
-    else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
+    else if (k == "--memory-preset") {
+      if (args[i + 1] == "max_capacity") flags |= MGBFS_FLAG_MAX_CAPACITY;
+      else if (args[i + 1] != "balanced") throw CliError("INVALID_MEMORY_PRESET");
+    } else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
   }
   if (cache && device != default_device) throw CliError("GENERIC_SESSION_DEVICE");
   const char* world = std::getenv("WORLD_SIZE");
@@ -320,7 +324,9 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
   opts.device = device;
   opts.backend = backend_code(backend);
   opts.capacity = capacity;
-  const std::string key = graph.digest() + "|" + std::to_string(device) + "|" + backend + "|" + std::to_string(capacity);
+  opts.flags = flags;
+  const std::string key = graph.digest() + "|" + std::to_string(device) + "|" + backend + "|" +
+                          std::to_string(capacity) + "|" + std::to_string(flags);
   void* engine = nullptr;
   char err[256] = {0};
   const auto setup0 = std::chrono::steady_clock::now();

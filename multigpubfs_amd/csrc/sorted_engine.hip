@@ -58,12 +58,7 @@ class SortedEngine final : public Engine {
     MGBFS_HIP(hipSetDevice(device_));
     MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto& e : ev_) MGBFS_HIP(hipEventCreate(&e));
-    budget_ = o.vram_budget;
-    if (!budget_) {
-      size_t fr = 0, tot = 0;
-      MGBFS_HIP(hipMemGetInfo(&fr, &tot));
-      budget_ = fr > (2ull << 30) ? fr - (2ull << 30) : 0;
-    }
+    budget_ = vram_budget(o);
     cap_limit_ = o.capacity ? o.capacity : ~0ull;
     flags_ = o.flags;
     // Stored-value layout and parent-edge tags.

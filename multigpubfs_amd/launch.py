@@ -58,8 +58,18 @@ def _error_code(stderr: str) -> str:
 
 
 def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_seconds=None, executable=None,
-              autotune=True, backend="auto", memory_backend="legacy", timeout=None):
-    """Run one exact BFS on one GPU and return the parsed report.json."""
+              shards=None, autotune=True, backend="auto", memory_backend="legacy", segment_rows=None,
+              memory_preset="balanced", segment_strategy=None, peer_transport="auto", transport="auto",
+              candidate_order="auto", bitmap_owner_page=None, timeout=None):
+    """Run one exact BFS on one GPU and return the parsed report.json.
+
+    Keyword arguments and their validation codes follow the reference
+    (launch.py:151-200).  Multi-rank and reference-internal tuning knobs
+    (shards, segment_rows, segment_strategy, transport, candidate_order,
+    bitmap_owner_page) are validated and otherwise have no effect: one GPU
+    runs the whole BFS and the engines size their own tables.
+    memory_preset="max_capacity" lowers the VRAM reserve and tightens slabs.
+    The specialized SHARD_AB backends are not part of this build."""
     if not isinstance(graph, GraphDefinition):
         graph = from_cayleypy(graph)
     if device is not None and devices is not None:
@@ -77,16 +87,51 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
     device = devices[0] if devices else 0
     if max_seconds is not None and (type(max_seconds) is not int or max_seconds < 0):
         raise ValueError("INVALID_MAX_SECONDS")
+    if shards is not None and (type(shards) is not int or not 1 <= shards <= 4096):
+        raise ValueError("INVALID_SHARDS")
     if type(autotune) is not bool:
         raise ValueError("INVALID_AUTOTUNE")
-    if backend not in ("auto", "generic"):
+    if segment_rows is not None and (type(segment_rows) is not int or not 1 <= segment_rows <= 1 << 32):
+        raise ValueError("INVALID_SEGMENT_ROWS")
+    if bitmap_owner_page is not None and (type(bitmap_owner_page) is not int or bitmap_owner_page < 0
+                                          or bitmap_owner_page > 1 << 20
+                                          or (bitmap_owner_page and bitmap_owner_page & (bitmap_owner_page - 1))):
+        raise ValueError("INVALID_BITMAP_OWNER_PAGE")
+    if backend not in ("auto", "generic", "shard_ab_hash", "shard_ab_sort_merge"):
         raise ValueError("INVALID_BACKEND")
+    if backend.startswith("shard_ab_"):
+        raise ValueError("SPECIALIZED_BACKEND_NOT_PROVIDED")
     if memory_backend not in _MEMORY_BACKENDS:
         raise ValueError("INVALID_MEMORY_BACKEND")
+    if bitmap_owner_page and memory_backend != "permutation_bitmap":
+        raise ValueError("BITMAP_OWNER_PAGE_REQUIRES_BITMAP")
+    if segment_strategy is None:
+        segment_strategy = os.environ.get("MGBFS_SEGMENT_STRATEGY", "atomic")
+    if segment_strategy not in ("atomic", "warp", "batch"):
+        raise ValueError("INVALID_SEGMENT_STRATEGY")
+    if memory_backend == "legacy" and segment_strategy != "atomic":
+        raise ValueError("SEGMENT_STRATEGY_REQUIRES_SEGMENTED_BACKEND")
+    if memory_preset not in ("balanced", "max_capacity"):
+        raise ValueError("INVALID_MEMORY_PRESET")
+    if memory_preset == "max_capacity" and memory_backend not in ("segmented", "segmented24", "permutation_bitmap"):
+        raise ValueError("MEMORY_PRESET_UNSUPPORTED_FOR_LEGACY")
     engine_backend = "generic" if backend == "generic" else _MEMORY_BACKENDS[memory_backend]
-    if engine_backend == "permutation_bitmap" and graph.action["kind"] != "permutation":
-        raise ValueError("BITMAP_REQUIRES_PERMUTATION_ACTION")
-    if capacity is not None and (type(capacity) is not int or not 1 <= capacity <= 0xFFFFFFFFFFFFFFFF):
+    if engine_backend == "permutation_bitmap":
+        if graph.action["kind"] != "permutation":
+            raise ValueError("BITMAP_REQUIRES_PERMUTATION_ACTION")
+        if not 1 <= len(graph.start) <= 64:
+            raise ValueError("BITMAP_UNSUPPORTED_DEGREE")
+    if peer_transport not in ("auto", "host", "lsa"):
+        raise ValueError("INVALID_PEER_TRANSPORT")
+    if peer_transport == "lsa":
+        raise ValueError("LSA_REQUIRES_SPECIALIZED_BACKEND")
+    if transport not in ("auto", "full", "parent"):
+        raise ValueError("INVALID_TRANSPORT")
+    if candidate_order not in ("auto", "none", "radix"):
+        raise ValueError("INVALID_CANDIDATE_ORDER")
+    wide = memory_backend in ("segmented", "segmented24", "permutation_bitmap")
+    if capacity is not None and (type(capacity) is not int
+                                 or not 1 <= capacity <= (0xFFFFFFFFFFFFFFFF if wide else 0x7FFFFFFE)):
         raise ValueError("INVALID_CAPACITY")
     output = Path(output)
     exe = native_runtime(executable)
@@ -98,6 +143,8 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
             cmd += ["--capacity", str(capacity)]
         if max_seconds:
             cmd += ["--seconds", str(max_seconds)]
+        if memory_preset == "max_capacity":
+            cmd += ["--memory-preset", "max_capacity"]
         from . import session
         pool = session.active_pool()
         if pool is not None:

@@ -207,6 +207,31 @@ def test_run_graph_argument_validation(tmp_path):
         run_graph(m, tmp_path / "out", memory_backend="permutation_bitmap")
 
 
+@pytest.mark.parametrize("kwargs,code", [
+    (dict(shards=0), "INVALID_SHARDS"),
+    (dict(shards=4097), "INVALID_SHARDS"),
+    (dict(segment_rows=0), "INVALID_SEGMENT_ROWS"),
+    (dict(segment_rows=(1 << 32) + 1), "INVALID_SEGMENT_ROWS"),
+    (dict(bitmap_owner_page=3), "INVALID_BITMAP_OWNER_PAGE"),
+    (dict(bitmap_owner_page=64), "BITMAP_OWNER_PAGE_REQUIRES_BITMAP"),
+    (dict(segment_strategy="fast"), "INVALID_SEGMENT_STRATEGY"),
+    (dict(segment_strategy="warp"), "SEGMENT_STRATEGY_REQUIRES_SEGMENTED_BACKEND"),
+    (dict(memory_preset="huge"), "INVALID_MEMORY_PRESET"),
+    (dict(memory_preset="max_capacity"), "MEMORY_PRESET_UNSUPPORTED_FOR_LEGACY"),
+    (dict(peer_transport="nvlink"), "INVALID_PEER_TRANSPORT"),
+    (dict(peer_transport="lsa"), "LSA_REQUIRES_SPECIALIZED_BACKEND"),
+    (dict(transport="half"), "INVALID_TRANSPORT"),
+    (dict(candidate_order="sorted"), "INVALID_CANDIDATE_ORDER"),
+    (dict(backend="shard_ab_hash"), "SPECIALIZED_BACKEND_NOT_PROVIDED"),
+    (dict(capacity=0x7FFFFFFF), "INVALID_CAPACITY"),  # legacy arenas use 31-bit rows
+])
+def test_run_graph_reference_keywords_validate_like_the_reference(tmp_path, kwargs, code):
+    # launch.py:151-200: same keyword names, validation order and codes.
+    from multigpubfs_amd import run_graph
+    with pytest.raises(ValueError, match=code):
+        run_graph(lrx(4), tmp_path / "out", **kwargs)
+
+
 # ------------------------------------------------------ command-line host
 
 def cli(*args):

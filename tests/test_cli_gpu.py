@@ -67,3 +67,11 @@ def test_output_directory_must_not_exist(tmp_path):
     with pytest.raises(NativeError) as err:
         run_graph(lrx(5), tmp_path / "x", device=0)
     assert err.value.code == "CLI_GRAPH_OUTPUT_EXISTS"
+
+
+@pytest.mark.parametrize("memory_backend", ["segmented", "permutation_bitmap"])
+def test_max_capacity_preset_runs_exact(tmp_path, memory_backend):
+    # memory_preset="max_capacity" keeps a 256 MiB VRAM reserve instead of 2 GiB.
+    report = run_graph(lrx(9), tmp_path / "out", device=0, memory_backend=memory_backend,
+                       memory_preset="max_capacity")
+    assert report["layer_sizes"] == O.packed_bfs(9)["layers"] and report["status"] == "COMPLETE"
