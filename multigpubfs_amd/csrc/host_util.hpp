@@ -270,7 +270,8 @@ class Sha256 {
       }
     }
   }
-  std::string hex() {
+  // Finishes the hash (call once).
+  void bytes(uint8_t out[32]) {
     uint8_t pad[72] = {0x80};
     const uint64_t bits = len_ * 8;
     const size_t padlen = (fill_ < 56) ? 56 - fill_ : 120 - fill_;
@@ -278,14 +279,18 @@ class Sha256 {
     for (int k = 0; k < 8; ++k) lenb[k] = static_cast<uint8_t>(bits >> (56 - 8 * k));
     update(pad, padlen);
     update(lenb, 8);
+    for (int w = 0; w < 8; ++w)
+      for (int k = 0; k < 4; ++k) out[4 * w + k] = static_cast<uint8_t>(h_[w] >> (24 - 8 * k));
+  }
+  std::string hex() {
+    uint8_t b[32];
+    bytes(b);
     static const char* digits = "0123456789abcdef";
     std::string out;
-    for (uint32_t w : h_)
-      for (int k = 3; k >= 0; --k) {
-        const uint8_t byte = static_cast<uint8_t>(w >> (8 * k));
-        out += digits[byte >> 4];
-        out += digits[byte & 15];
-      }
+    for (uint8_t byte : b) {
+      out += digits[byte >> 4];
+      out += digits[byte & 15];
+    }
     return out;
   }
 

@@ -19,8 +19,10 @@
 #include <chrono>
 #include <csignal>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -29,9 +31,11 @@
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "../../include/mgbfs_hip.h"
+#include "archive.hpp"
 #include "host_util.hpp"
 
 using mgbfs::host::Json;
+using mgbfs::host::LayerArchive;
 
 namespace {
 
@@ -292,10 +296,13 @@ bool path_exists(const std::string& path) {
 Json run_graph_job(const std::vector<std::string>& args, int32_t default_device, EngineCache* cache,
                    const std::string& cancel_path, bool* reused) {
   if (args.size() < 2)
-    throw CliError("CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N]");
+    throw CliError(
+        "CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N] "
+        "[--archive PATH]");
   int32_t device = default_device;
   uint64_t capacity = 0, seconds = 0;
   std::string backend = "auto";
+  std::string archive_path;
   uint32_t flags = 0;
   for (size_t i = 2; i < args.size(); i += 2) {
     if (i + 1 >= args.size()) throw CliError("CLI_GRAPH_OPTION_VALUE");
@@ -304,6 +311,7 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     else if (k == "--capacity") capacity = parse_u64(args[i + 1], "CLI_GRAPH_CAPACITY");
     else if (k == "--seconds") seconds = parse_u64(args[i + 1], "CLI_GRAPH_SECONDS");
     else if (k == "--backend") backend = args[i + 1];
+    else if (k == "--archive") archive_path = args[i + 1];
     else if (k == "--memory-preset") {
       if (args[i + 1] == "max_capacity") flags |= MGBFS_FLAG_MAX_CAPACITY;
       else if (args[i + 1] != "balanced") throw CliError("INVALID_MEMORY_PRESET");
@@ -350,6 +358,18 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
   std::signal(SIGTERM, on_signal);
   std::signal(SIGINT, on_signal);
   std::cerr << "MGBFS_GRAPH_READY device=" << device << std::endl;
+  std::unique_ptr<LayerArchive> archive;
+  std::vector<int64_t> arows(1000 * static_cast<size_t>(graph.elements));
+  auto archive_layer = [&](uint64_t depth) {
+    uint64_t n = 0;
+    if (mgbfs_engine_sample(engine, 0, 1000, arows.data(), &n)) throw CliError("ARCHIVE_SAMPLE");
+    archive->layer(depth, arows, n);
+  };
+  if (!archive_path.empty()) {
+    if (graph.kind != 0) throw CliError("ARCHIVE_STATE_WIDTH");
+    archive = std::make_unique<LayerArchive>(archive_path, graph.elements, graph.digest());
+    archive_layer(0);
+  }
   std::vector<double> times;
   std::string status = "COMPLETE", reason = "FRONTIER_EXHAUSTED";
   const auto t0 = std::chrono::steady_clock::now();
@@ -375,6 +395,7 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     }
     if (s == MGBFS_LAYER) {
       times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count());
+      if (archive) archive_layer(times.size());
       continue;
     }
     if (s == MGBFS_RESOURCE) {
@@ -384,6 +405,7 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     break;
   }
   const double bfs_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (archive) archive->commit();
   uint32_t nl = 0;
   mgbfs_engine_layer_sizes(engine, nullptr, 0, &nl);
   std::vector<uint64_t> sizes(nl);

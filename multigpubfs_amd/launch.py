@@ -28,6 +28,10 @@ _MEMORY_BACKENDS = {
 }
 
 
+# Same file name as the reference's rank archives (specialized.py:164).
+ARCHIVE_NAME = "archive-rank-0.mgbfsar1"
+
+
 class NativeError(RuntimeError):
     """The native executable failed; .code is its ERROR code."""
 
@@ -60,7 +64,7 @@ def _error_code(stderr: str) -> str:
 def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_seconds=None, executable=None,
               shards=None, autotune=True, backend="auto", memory_backend="legacy", segment_rows=None,
               memory_preset="balanced", segment_strategy=None, peer_transport="auto", transport="auto",
-              candidate_order="auto", bitmap_owner_page=None, timeout=None):
+              candidate_order="auto", bitmap_owner_page=None, timeout=None, archive=False):
     """Run one exact BFS on one GPU and return the parsed report.json.
 
     Keyword arguments and their validation codes follow the reference
@@ -69,7 +73,14 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
     bitmap_owner_page) are validated and otherwise have no effect: one GPU
     runs the whole BFS and the engines size their own tables.
     memory_preset="max_capacity" lowers the VRAM reserve and tightens slabs.
-    The specialized SHARD_AB backends are not part of this build."""
+    The specialized SHARD_AB backends are not part of this build.
+
+    archive=True also writes output/archive-rank-0.mgbfsar1, the layer
+    archive (MGBFSAS2: up to 1000 states of every layer, one byte per
+    element, sha256-chained frames; permutation graphs only), checks it the
+    way the reference checks its rank archives (specialized.py:187-190:
+    config digest = graph digest, one archived layer per reported layer) and
+    returns its receipt under report["archive"]."""
     if not isinstance(graph, GraphDefinition):
         graph = from_cayleypy(graph)
     if device is not None and devices is not None:
@@ -133,6 +144,10 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
     if capacity is not None and (type(capacity) is not int
                                  or not 1 <= capacity <= (0xFFFFFFFFFFFFFFFF if wide else 0x7FFFFFFE)):
         raise ValueError("INVALID_CAPACITY")
+    if type(archive) is not bool:
+        raise ValueError("INVALID_ARCHIVE")
+    if archive and graph.action["kind"] != "permutation":
+        raise ValueError("ARCHIVE_REQUIRES_PERMUTATION_ACTION")
     output = Path(output)
     exe = native_runtime(executable)
     with tempfile.TemporaryDirectory(prefix="mgbfs-graph-") as tmp:
@@ -145,6 +160,8 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
             cmd += ["--seconds", str(max_seconds)]
         if memory_preset == "max_capacity":
             cmd += ["--memory-preset", "max_capacity"]
+        if archive:
+            cmd += ["--archive", str(output / ARCHIVE_NAME)]
         from . import session
         pool = session.active_pool()
         if pool is not None:
@@ -156,4 +173,14 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
             proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
             if proc.returncode:
                 raise NativeError(_error_code(proc.stderr), proc.stderr)
-    return json.loads((output / "report.json").read_text())
+    report = json.loads((output / "report.json").read_text())
+    if archive:
+        from .archive import read_archive
+        receipt = read_archive(output / ARCHIVE_NAME, len(graph.start))
+        if receipt["config_digest"] != report["graph_digest"]:
+            raise RuntimeError("SPECIALIZED_ARCHIVE_IDENTITY_MISMATCH")
+        if receipt["depths"] != len(report["layer_sizes"]):
+            raise RuntimeError("SPECIALIZED_LAYER_CONSENSUS")
+        report["archive"] = dict(path=str(output / ARCHIVE_NAME), format="MGBFSAS2", depths=receipt["depths"],
+                                 terminal_replaced=receipt["terminal_replaced"])
+    return report

@@ -205,6 +205,10 @@ def test_run_graph_argument_validation(tmp_path):
     m = GraphDefinition.matrix(1, 1, [([2], 5)], [1])
     with pytest.raises(ValueError, match="BITMAP_REQUIRES_PERMUTATION_ACTION"):
         run_graph(m, tmp_path / "out", memory_backend="permutation_bitmap")
+    with pytest.raises(ValueError, match="ARCHIVE_REQUIRES_PERMUTATION_ACTION"):
+        run_graph(m, tmp_path / "out", archive=True)
+    with pytest.raises(ValueError, match="INVALID_ARCHIVE"):
+        run_graph(lrx(4), tmp_path / "out", archive="yes")
 
 
 @pytest.mark.parametrize("kwargs,code", [

@@ -75,3 +75,29 @@ def test_max_capacity_preset_runs_exact(tmp_path, memory_backend):
     report = run_graph(lrx(9), tmp_path / "out", device=0, memory_backend=memory_backend,
                        memory_preset="max_capacity")
     assert report["layer_sizes"] == O.packed_bfs(9)["layers"] and report["status"] == "COMPLETE"
+
+
+@pytest.mark.parametrize("memory_backend", ["segmented", "permutation_bitmap", "generic"])
+def test_archive_samples_every_layer(tmp_path, memory_backend):
+    from multigpubfs_amd import archive as A
+    from multigpubfs_amd.launch import ARCHIVE_NAME
+    n = 9
+    out = tmp_path / "a"
+    report = run_graph(lrx(n), out, device=0, memory_backend=memory_backend, archive=True)
+    ref = O.packed_bfs(n, keep_keys=True)
+    assert report["layer_sizes"] == ref["layers"]
+    assert report["archive"]["depths"] == len(ref["layers"]) and not report["archive"]["terminal_replaced"]
+    data = (out / ARCHIVE_NAME).read_bytes()
+    assert A.verify(data)["depths"] == len(ref["layers"])
+    walked = list(A.layers(data))
+    assert [d for d, _ in walked] == list(range(len(ref["layers"])))
+    for d, raw in walked:
+        keys = [O.pack(list(raw[i:i + n])) for i in range(0, len(raw), n)]
+        assert len(keys) == min(1000, ref["layers"][d]) and len(set(keys)) == len(keys)
+        assert set(keys) <= set(ref["keys"][d])
+
+
+def test_archive_of_resource_stop_commits_finished_layers(tmp_path):
+    report = run_graph(lrx(8), tmp_path / "r", device=0, capacity=50, memory_backend="generic", archive=True)
+    assert report["status"] == "INCOMPLETE"
+    assert report["archive"]["depths"] == len(report["layer_sizes"])
