@@ -35,26 +35,49 @@ struct PackedGenerators {
   // The same permutation as shift groups: child nibble i takes parent nibble
   // i + delta for every destination nibble in the group's mask, so a
   // generator costs one shift and one mask per distinct displacement
-  // (2 for a rotation, 3 for a transposition of neighbours).
+  // (2 for a rotation, 3 for a transposition of neighbours).  Groups with
+  // delta >= 0 (right shifts) come first, `right` of them; the shift
+  // amounts 4|delta| are packed four to a word so that a uniform group
+  // index reads them with one scalar load.
   uint8_t groups[kMaxGenerators];
-  int8_t delta[kMaxGenerators][kMaxDegree];
+  uint8_t right[kMaxGenerators];
+  uint32_t shift4[kMaxGenerators][kMaxDegree / 4];
   uint64_t gmask[kMaxGenerators][kMaxDegree];
 };
 
+__host__ __device__ inline uint32_t group_shift(const PackedGenerators& pg, int g, uint32_t j) {
+  return (pg.shift4[g][j >> 2] >> (8u * (j & 3u))) & 0xffu;
+}
+
 __host__ inline void fill_shift_groups(PackedGenerators& pg) {
   for (uint32_t g = 0; g < pg.count; ++g) {
-    pg.groups[g] = 0;
+    int delta[kMaxDegree];
+    uint64_t mask[kMaxDegree];
+    uint32_t groups = 0;
     for (uint32_t i = 0; i < pg.degree; ++i) {
       const int d = static_cast<int>(pg.src[g][i]) - static_cast<int>(i);
       uint32_t j = 0;
-      while (j < pg.groups[g] && pg.delta[g][j] != d) ++j;
-      if (j == pg.groups[g]) {
-        pg.delta[g][j] = static_cast<int8_t>(d);
-        pg.gmask[g][j] = 0;
-        ++pg.groups[g];
+      while (j < groups && delta[j] != d) ++j;
+      if (j == groups) {
+        delta[j] = d;
+        mask[j] = 0;
+        ++groups;
       }
-      pg.gmask[g][j] |= 15ull << (4 * i);
+      mask[j] |= 15ull << (4 * i);
     }
+    pg.groups[g] = static_cast<uint8_t>(groups);
+    pg.right[g] = 0;
+    for (uint32_t w = 0; w < kMaxDegree / 4; ++w) pg.shift4[g][w] = 0;
+    uint32_t out = 0;
+    for (int pass = 0; pass < 2; ++pass)  // right shifts (delta >= 0) first
+      for (uint32_t j = 0; j < groups; ++j) {
+        if ((delta[j] >= 0) != (pass == 0)) continue;
+        const uint32_t sh = 4u * static_cast<uint32_t>(delta[j] >= 0 ? delta[j] : -delta[j]);
+        pg.shift4[g][out >> 2] |= sh << (8u * (out & 3u));
+        pg.gmask[g][out] = mask[j];
+        if (pass == 0) ++pg.right[g];
+        ++out;
+      }
   }
 }
 
@@ -88,16 +111,35 @@ __device__ __forceinline__ uint64_t apply_packed(uint64_t key, const uint8_t* sr
   return (static_cast<uint64_t>(chi) << 32) | clo;
 }
 
-// Same result as apply_packed, from the shift groups of generator g.
-__device__ __forceinline__ uint64_t apply_groups(uint64_t key, const PackedGenerators& pg, int g) {
-  uint64_t child = 0;
-  const uint32_t groups = pg.groups[g];
-  for (uint32_t j = 0; j < groups; ++j) {
-    const int d = pg.delta[g][j];
-    const uint64_t moved = d >= 0 ? (key >> (4 * d)) : (key << (-4 * d));
-    child |= moved & pg.gmask[g][j];
+// Same result as apply_packed, from the shift groups of generator g, for N
+// keys at once: each group's shift and mask are uniform (scalar loads) and
+// are applied to all N keys before the next group is read.
+template <int N>
+__device__ __forceinline__ void apply_groups_n(const uint64_t (&key)[N], uint64_t (&child)[N],
+                                               const PackedGenerators& pg, int g) {
+#pragma unroll
+  for (int m = 0; m < N; ++m) child[m] = 0;
+  const uint32_t groups = pg.groups[g], right = pg.right[g];
+  uint32_t j = 0;
+  for (; j < right; ++j) {
+    const uint32_t sh = group_shift(pg, g, j);
+    const uint64_t mask = pg.gmask[g][j];
+#pragma unroll
+    for (int m = 0; m < N; ++m) child[m] |= (key[m] >> sh) & mask;
   }
-  return child;
+  for (; j < groups; ++j) {
+    const uint32_t sh = group_shift(pg, g, j);
+    const uint64_t mask = pg.gmask[g][j];
+#pragma unroll
+    for (int m = 0; m < N; ++m) child[m] |= (key[m] << sh) & mask;
+  }
+}
+
+__device__ __forceinline__ uint64_t apply_groups(uint64_t key, const PackedGenerators& pg, int g) {
+  const uint64_t k[1] = {key};
+  uint64_t c[1];
+  apply_groups_n<1>(k, c, pg, g);
+  return c[0];
 }
 
 __host__ inline uint64_t apply_packed_host(uint64_t key, const uint8_t* src, uint32_t n) {

@@ -87,16 +87,6 @@ __device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t c
 
 // ------------------------------------------------------------ expansion
 
-// Stored value of a child made by generator g from `key`.  With back-edge
-// masks the tag is the bit of g's inverse (child_tag[g] = inverse + 1): the
-// child's generator that leads back to this parent.
-__device__ __forceinline__ uint64_t child_value(uint64_t key, const PackedGenerators& gens, int g,
-                                                const SortedLayout& L) {
-  const uint64_t child = apply_groups(key, gens, g);
-  const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
-  return (keymix(child, L.km) << L.tagbits) | tag;
-}
-
 // Shared tile machinery.  Each thread holds up to PER tile elements in
 // registers (bit k of `valid` marks v[k]); they are ranked within their
 // bucket by LDS atomics and placed bucket-contiguously in t.s.  reserve(b, n)
@@ -208,9 +198,10 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     }
     uint64_t v[T::kRegs];
     uint32_t valid = 0;
+    uint64_t key[T::kParentsPerThread];
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
-      const uint64_t key = keyunmix(pv[m] >> L.tagbits, L.km);
+      key[m] = keyunmix(pv[m] >> L.tagbits, L.km);
       uint32_t skip = MASKED ? static_cast<uint32_t>(pv[m] & L.tagmask) : 0u;
       if (MASKED && live[m] && skip == 0) {
         // A discovered state always has a back-edge; an empty mask would
@@ -219,10 +210,20 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
         skip = (1u << G) - 1u;
       }
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        v[m * G + g] = child_value(key, gens, g, L);
+      for (int g = 0; g < G; ++g)
         if (live[m] && !((skip >> g) & 1u)) valid |= 1u << (m * G + g);
-      }
+    }
+    // Generator-major: each shift group is read once and applied to all of
+    // this thread's parents.
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      uint64_t child[T::kParentsPerThread];
+      apply_groups_n(key, child, gens, g);
+      // With back-edge masks the child's tag is the bit of g's inverse
+      // (child_tag[g] = inverse + 1): the generator leading back here.
+      const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
+#pragma unroll
+      for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
     }
     route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
   }
