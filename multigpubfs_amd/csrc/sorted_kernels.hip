@@ -39,8 +39,10 @@ constexpr int kUniqueThreads = 512;
 constexpr int kUniquePerThread = kUniqueCap / kUniqueThreads;  // candidates held in registers
 constexpr int kPrefetch = 4;  // independent loads issued before dependent LDS work
 
+// Top `bits` (0..31) bits of h, from the high word with two 32-bit shifts
+// (no select for bits = 0).
 __device__ __forceinline__ uint32_t top_bits(uint64_t h, uint32_t bits) {
-  return bits ? static_cast<uint32_t>(h >> (64u - bits)) : 0u;
+  return (static_cast<uint32_t>(h >> 32) >> 1) >> (31u - bits);
 }
 
 __device__ __forceinline__ uint32_t mid_bits(uint64_t h, uint32_t skip, uint32_t bits) {
@@ -188,31 +190,34 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
   const uint64_t tiles = (F + T::kParents - 1) / T::kParents;
   for (uint64_t tile = blockIdx.x; tile < tiles; tile += NB) {
     const uint64_t t0 = tile * T::kParents;
+    // Loads past the frontier re-read its last parent (F >= 1), so every
+    // load is in bounds and none is predicated; `live` drops them.
     uint64_t pv[T::kParentsPerThread];
-    bool live[T::kParentsPerThread];
+    uint32_t live = 0;
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
       const uint64_t q = t0 + static_cast<uint64_t>(m) * kRouteThreads + threadIdx.x;
-      live[m] = q < F;
-      pv[m] = live[m] ? bank[q] : 0;
+      live |= (q < F ? 1u : 0u) << m;
+      pv[m] = bank[q < F ? q : F - 1];
     }
     uint64_t v[T::kRegs];
     uint32_t valid = 0;
+    bool empty_mask = false;
     uint64_t key[T::kParentsPerThread];
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
       key[m] = keyunmix(pv[m] >> L.tagbits, L.km);
-      uint32_t skip = MASKED ? static_cast<uint32_t>(pv[m] & L.tagmask) : 0u;
-      if (MASKED && live[m] && skip == 0) {
+      uint32_t want = (1u << G) - 1u;
+      if (MASKED) {
+        const uint32_t skip = static_cast<uint32_t>(pv[m] & L.tagmask);
         // A discovered state always has a back-edge; an empty mask would
         // let G children into a tile sized for G-1 per parent.
-        atomicOr(error, kErrDomain);
-        skip = (1u << G) - 1u;
+        empty_mask |= ((live >> m) & 1u) && skip == 0;
+        want = skip == 0 ? 0u : (~skip & want);
       }
-#pragma unroll
-      for (int g = 0; g < G; ++g)
-        if (live[m] && !((skip >> g) & 1u)) valid |= 1u << (m * G + g);
+      valid |= ((live >> m) & 1u ? want : 0u) << (m * G);
     }
+    if (MASKED && empty_mask) atomicOr(error, kErrDomain);
     // Generator-major: each shift group is read once and applied to all of
     // this thread's parents.
 #pragma unroll
