@@ -36,15 +36,20 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
 // (lane * P + blockIdx % P) of `cap` entries, `stride` entries apart,
 // cursors[] counts per slab
 // (zeroed by the caller); a run beyond `cap` is dropped and sets err_bit.
+// Routing outputs: `out` holds out_entries slab entries followed by a dump
+// area of kRouteDump entries (a slab overflow is written there; the layer
+// is then redone).
+constexpr uint64_t kRouteDump = 8192;
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
                                bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
-                               hipStream_t stream);
+                               uint64_t out_entries, hipStream_t stream);
 // Lane slabs (lcount per slab, capacity and stride lcap) -> sub-bucket
 // slabs of kUniqueCap entries, kSlabStride apart (scount per sub-bucket,
 // zeroed by the caller).
 hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
-                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, hipStream_t stream);
+                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, uint64_t out_entries,
+                            hipStream_t stream);
 // In-layer dedup of the sub-bucket slabs and difference against the
 // current bank and, when given, the previous one; survivors are written back
 // grouped by Ko bits.  merge: OR duplicates' back-edge masks.

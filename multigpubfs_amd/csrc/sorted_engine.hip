@@ -163,7 +163,8 @@ class SortedEngine final : public Engine {
       NB1 = (NB1 + P - 1) / P * P;
       const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(Cest) / ((1ull << K1) * P), slack_) : kUniqueCap;
       const uint64_t lslabs = (1ull << K1) * P;
-      if (!ensure_slabs(K2 ? lslabs * lcap : 0, (1ull << K) * kSlabStride)) {
+      const uint64_t lane_entries = K2 ? lslabs * lcap : 0, sub_entries = (1ull << K) * kSlabStride;
+      if (!ensure_slabs(lane_entries + kRouteDump, sub_entries + kRouteDump)) {
         *fatal = kErrFutureCap;
         terminal = true;
         return MGBFS_RESOURCE;
@@ -180,11 +181,11 @@ class SortedEngine final : public Engine {
       MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, K1, NB1, P, lcap,
                                     K2 ? lcap : kSlabStride, K2 ? kErrLaneSlab : kErrBucketCap,
                                     K2 ? lcount_.as<uint32_t>() : scount_.as<uint32_t>(), error_word(),
-                                    K2 ? lane_.as<uint64_t>() : sub, stream_));
+                                    K2 ? lane_.as<uint64_t>() : sub, K2 ? lane_entries : sub_entries, stream_));
       MGBFS_HIP(hipEventRecord(ev_[1], stream_));
       if (K2)
         MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, K1, K2,
-                                   scount_.as<uint32_t>(), error_word(), sub, stream_));
+                                   scount_.as<uint32_t>(), error_word(), sub, sub_entries, stream_));
       MGBFS_HIP(hipEventRecord(ev_[2], stream_));
       auto* control = reinterpret_cast<unsigned long long*>(control_.as<uint64_t>());
       MGBFS_HIP(sorted_unique_diff(

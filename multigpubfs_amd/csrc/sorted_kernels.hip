@@ -35,6 +35,7 @@ namespace mgbfs {
 constexpr int kRouteThreads = 512;
 constexpr int kTile = 8192;                            // elements per LDS routing tile
 constexpr int kTilePerThread = kTile / kRouteThreads;  // 16 elements held in registers
+static_assert(kRouteDump >= kTile, "a dropped run fits the dump area");
 constexpr int kUniqueThreads = 512;
 constexpr int kUniquePerThread = kUniqueCap / kUniqueThreads;  // candidates held in registers
 constexpr int kPrefetch = 4;  // independent loads issued before dependent LDS work
@@ -106,10 +107,19 @@ struct TileLds {
   uint32_t wave_sums[kRouteThreads / kWave];
 };
 
+// Destination indexing.  When the output array (plus a kTile-entry dump
+// area at index `dump`) has fewer than 2^32 entries, t.off[b] holds bucket
+// b's 32-bit destination index minus the run's LDS start, so the write-out
+// is one add per element; a dropped run is pointed into the dump area (the
+// layer is redone anyway, the writes only have to stay in bounds).
+// Otherwise (dump = kWideOut) t.off[b] is the run's slab offset and the
+// write-out computes 64-bit addresses and skips dropped runs.
+constexpr uint32_t kWideOut = ~0u;
+
 template <int PER, typename BucketFn, typename ReserveFn, typename BaseFn>
 __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid, uint32_t buckets,
                                            BucketFn bkt, ReserveFn reserve, BaseFn slab_base,
-                                           uint64_t* __restrict__ out) {
+                                           uint64_t* __restrict__ out, uint32_t dump) {
   uint32_t rank[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) rank[k] = ((valid >> k) & 1u) ? atomicAdd(&t.lhist[bkt(v[k])], 1u) : 0u;
@@ -124,14 +134,29 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
 #pragma unroll
   for (int k = 0; k < PER; ++k)
     if ((valid >> k) & 1u) t.s[t.lstart[bkt(v[k])] + rank[k]] = v[k];
-  if (b0 < buckets) t.off[b0] = off0;
-  if (b1 < buckets) t.off[b1] = off1;
+  if (dump != kWideOut) {
+    auto dest = [&](uint32_t b, uint32_t off) {
+      return (off == kDropRun ? dump : static_cast<uint32_t>(slab_base(b)) + off) - t.lstart[b];
+    };
+    if (b0 < buckets) t.off[b0] = dest(b0, off0);
+    if (b1 < buckets) t.off[b1] = dest(b1, off1);
+  } else {
+    if (b0 < buckets) t.off[b0] = off0;
+    if (b1 < buckets) t.off[b1] = off1;
+  }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
-    const uint64_t h = t.s[j];
-    const uint32_t b = bkt(h);
-    const uint32_t off = t.off[b];
-    if (off != kDropRun) out[slab_base(b) + off + (j - t.lstart[b])] = h;
+  if (dump != kWideOut) {
+    for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
+      const uint64_t h = t.s[j];
+      out[t.off[bkt(h)] + j] = h;  // 32-bit index arithmetic, wraps as intended
+    }
+  } else {
+    for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
+      const uint64_t h = t.s[j];
+      const uint32_t b = bkt(h);
+      const uint32_t off = t.off[b];
+      if (off != kDropRun) out[slab_base(b) + off + (j - t.lstart[b])] = h;
+    }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
@@ -172,7 +197,8 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
                                                                         uint32_t err_bit,
                                                                         uint32_t* __restrict__ cursors,
                                                                         uint32_t* __restrict__ error,
-                                                                        uint64_t* __restrict__ out) {
+                                                                        uint64_t* __restrict__ out,
+                                                                        uint32_t dump) {
   using T = ExpandTile<G, MASKED>;
   static_assert(T::kRegs <= 32, "valid mask is 32 bits");
   __shared__ TileLds t;
@@ -230,7 +256,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 #pragma unroll
       for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
     }
-    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
+    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
   }
 }
 
@@ -245,7 +271,8 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
                                                                            uint32_t K2,
                                                                            uint32_t* __restrict__ scount,
                                                                            uint32_t* __restrict__ error,
-                                                                           uint64_t* __restrict__ out) {
+                                                                           uint64_t* __restrict__ out,
+                                                                           uint32_t dump) {
   __shared__ TileLds t;
   const uint32_t buckets = 1u << K2;
   for (uint32_t b = threadIdx.x; b < buckets; b += kRouteThreads) t.lhist[b] = 0;
@@ -267,7 +294,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
       v[k] = i < r1 ? in[i] : 0;
       if (i < r1) valid |= 1u << k;
     }
-    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out);
+    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
   }
 }
 
@@ -569,29 +596,35 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
   return hipGetLastError();
 }
 
+// Dump-area index for 32-bit destination indexing, or kWideOut when the
+// output (entries, plus kTile dump entries) does not fit 32-bit indices.
+static uint32_t dump_index(uint64_t out_entries) {
+  return out_entries + kTile < kWideOut ? static_cast<uint32_t>(out_entries) : kWideOut;
+}
+
 template <int G, bool MASKED>
 static hipError_t expand_route_g(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
                                  uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                  uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
-                                 hipStream_t stream) {
+                                 uint64_t out_entries, hipStream_t stream) {
   hipLaunchKernelGGL((expand_scatter_kernel<G, MASKED>), dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L,
-                     K1, NB, P, cap, stride, err_bit, cursors, error, out);
+                     K1, NB, P, cap, stride, err_bit, cursors, error, out, dump_index(out_entries));
   return hipGetLastError();
 }
 
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
                                bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
-                               hipStream_t stream) {
+                               uint64_t out_entries, hipStream_t stream) {
   if (K1 > kMaxPassBits || NB == 0 || P == 0 || cap == 0 || stride < cap) return hipErrorInvalidValue;
   switch (gens.count) {
 #define MGBFS_G(k)                                                                                               \
   case k:                                                                                                        \
     return masked                                                                                                \
                ? expand_route_g<k, true>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
-                                         stream)                                                            \
+                                         out_entries, stream)                                               \
                : expand_route_g<k, false>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
-                                          stream);
+                                          out_entries, stream);
     MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
     MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
 #undef MGBFS_G
@@ -601,11 +634,12 @@ hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGen
 }
 
 hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
-                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, hipStream_t stream) {
+                            uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, uint64_t out_entries,
+                            hipStream_t stream) {
   if (K2 == 0 || K2 > kMaxPassBits || K1 + K2 > kMaxBucketBits || P == 0 || lcap == 0) return hipErrorInvalidValue;
   const uint32_t blocks = (1u << K1) * P;
   hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, lcount, lcap, P, K1,
-                     K2, scount, error, out);
+                     K2, scount, error, out, dump_index(out_entries));
   return hipGetLastError();
 }
 
