@@ -59,6 +59,8 @@ extern "C" {
                                              child is generated and both banks are probed */
 #define MGBFS_FLAG_MAX_CAPACITY 16u       /* keep 256 MiB of free VRAM in reserve instead of
                                              2 GiB (reference memory_preset "max_capacity") */
+#define MGBFS_FLAG_TEST_WIDE_ROUTING 32u  /* sorted backend: route with 64-bit slab addresses,
+                                             the path taken by outputs of 2^32 entries or more */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are

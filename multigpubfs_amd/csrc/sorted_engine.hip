@@ -181,11 +181,12 @@ class SortedEngine final : public Engine {
       MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, K1, NB1, P, lcap,
                                     K2 ? lcap : kSlabStride, K2 ? kErrLaneSlab : kErrBucketCap,
                                     K2 ? lcount_.as<uint32_t>() : scount_.as<uint32_t>(), error_word(),
-                                    K2 ? lane_.as<uint64_t>() : sub, K2 ? lane_entries : sub_entries, stream_));
+                                    K2 ? lane_.as<uint64_t>() : sub, routed(K2 ? lane_entries : sub_entries),
+                                    stream_));
       MGBFS_HIP(hipEventRecord(ev_[1], stream_));
       if (K2)
         MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, K1, K2,
-                                   scount_.as<uint32_t>(), error_word(), sub, sub_entries, stream_));
+                                   scount_.as<uint32_t>(), error_word(), sub, routed(sub_entries), stream_));
       MGBFS_HIP(hipEventRecord(ev_[2], stream_));
       auto* control = reinterpret_cast<unsigned long long*>(control_.as<uint64_t>());
       MGBFS_HIP(sorted_unique_diff(
@@ -286,6 +287,11 @@ class SortedEngine final : public Engine {
 
  private:
   uint32_t* error_word() { return reinterpret_cast<uint32_t*>(control_.as<uint64_t>() + 1); }
+  // Routing output size as passed to the kernels; the test flag reports an
+  // oversized output so they take their 64-bit addressing path.
+  uint64_t routed(uint64_t entries) const {
+    return (flags_ & MGBFS_FLAG_TEST_WIDE_ROUTING) ? ~0ull : entries;
+  }
   int next_bank() const {
     for (int b = 0; b < 3; ++b)
       if (b != cur_ && b != prev_) return b;

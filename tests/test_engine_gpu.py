@@ -130,6 +130,20 @@ def test_sorted_without_pruning_is_exact(n):
     assert info["candidates"] == 3 * math.factorial(n)
 
 
+@pytest.mark.parametrize("flags", [32, 32 | 4, 32 | 1])
+def test_s11_sorted_wide_routing_path_is_exact(flags):
+    # 64-bit slab addressing (outputs of 2^32 entries or more), alone and with
+    # the lane-slab and sub-bucket overflow retries, whose dropped runs that
+    # path skips instead of sending to the dump area.
+    ref = O.packed_bfs(11)["layers"]
+    with NativeEngine(lrx(11), backend="sorted", flags=flags) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == ref
+    if flags != 32:
+        assert info["retries"] > 0
+
+
 def test_s11_sorted_lane_slab_retry_path_is_exact():
     # Owner-lane slabs sized to the mean overflow; the layer is redone with slack.
     ref = O.packed_bfs(11)["layers"]
