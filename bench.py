@@ -165,6 +165,11 @@ def main(argv=None):
         gbs = phase_b[i] / phase_s[i] / 1e9
         phases.append({"kernel": name, "ms_per_step": 1e3 * phase_s[i] / args.steps,
                        "bytes_per_step": phase_b[i] // args.steps, "achieved_gbs": gbs, "bytes_model": models[i]})
+    if not phases:  # backends without per-phase events: all their kernels as one phase
+        phases.append({"kernel": f"{engine.backend_name} kernels", "ms_per_step": 1e3 * kernel_s / args.steps,
+                       "bytes_per_step": algo_bytes // args.steps,
+                       "achieved_gbs": algo_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0,
+                       "bytes_model": "the engine's streaming-bytes model"})
     dom = max(phases, key=lambda p: p["ms_per_step"])
     traffic, traffic_source = None, None
     tf = TRAFFIC_FILES.get("bitmap" if info["backend"] == 1 else "sorted")
