@@ -81,6 +81,35 @@ def serving_section(count: int) -> dict:
     return out
 
 
+def s11_bitmap_section(golden_all, steps=5):
+    """S11 LRX on the Lehmer-rank permutation bitmap (39,916,800 states):
+    mean of `steps` full BFS after one warmup.  Each layer table is checked
+    against the golden table when the golden file holds one, else against
+    the sorted backend's table for the same graph (an independent visited
+    set: hash + sort-unique instead of the rank bitmap) and 11! states."""
+    from multigpubfs_amd import NativeEngine, lrx
+    n = 11
+    states = math.factorial(n)
+    golden = golden_all.get(str(n))
+    if golden is None:
+        with NativeEngine(lrx(n), backend="sorted") as ref:
+            ref.run()
+            golden = ref.layer_sizes() if sum(ref.layer_sizes()) == states else None
+    with NativeEngine(lrx(n), backend="permutation_bitmap") as e:
+        e.run()
+        exact = (e.layer_sizes() == golden) if golden else sum(e.layer_sizes()) == states
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.reset()
+            e.run()
+            sizes = e.layer_sizes()
+            exact &= (sizes == golden) if golden else sum(sizes) == states
+        dt = (time.perf_counter() - t0) / steps
+    return {"workload": "S11 LRX full exact BFS, permutation bitmap", "states": states, "layers": len(sizes),
+            "ms_per_bfs": 1e3 * dt, "edges_per_s": 3 * states / dt, "layer_table_exact": bool(exact),
+            "checked_against": "golden table" if str(n) in golden_all else "sorted backend's table"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +120,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 256-instance batched sweep")
+    ap.add_argument("--no-bitmap", action="store_true", help="skip the S11 permutation-bitmap run")
     ap.add_argument("--no-serve", action="store_true", help="skip the 2,000-job serving run")
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--calibrate", action="store_true",
@@ -111,7 +141,8 @@ def main(argv=None):
         print("[bench] calibration copy: 1073741824 bytes read + written", file=sys.stderr, flush=True)
         del src, dst
     from multigpubfs_amd import NativeEngine, lrx
-    golden = json.loads((ROOT / "tests" / "golden" / "lrx_layers.json").read_text())["layers"].get(str(args.n))
+    golden_all = json.loads((ROOT / "tests" / "golden" / "lrx_layers.json").read_text())["layers"]
+    golden = golden_all.get(str(args.n))
     n = args.n
     states = math.factorial(n)
     edges = 3 * states
@@ -210,6 +241,9 @@ def main(argv=None):
                      "phases": phases},
     }
     engine.close()
+    if not args.no_bitmap:
+        result["s11_bitmap"] = s11_bitmap_section(golden_all)
+        print(f"[bench] s11 bitmap {result['s11_bitmap']}", file=sys.stderr, flush=True)
     if not args.no_sweep:
         result["sweep"] = sweep_section()
         print(f"[bench] sweep {result['sweep']}", file=sys.stderr, flush=True)
@@ -219,7 +253,8 @@ def main(argv=None):
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
-    ok = exact and result.get("sweep", {}).get("golden_exact", True) and not result.get("serving", {}).get("errors")
+    ok = (exact and result.get("sweep", {}).get("golden_exact", True) and not result.get("serving", {}).get("errors")
+          and result.get("s11_bitmap", {}).get("layer_table_exact", True))
     return 0 if ok else 1
 
 
