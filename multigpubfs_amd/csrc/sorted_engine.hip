@@ -31,7 +31,7 @@ uint32_t ceil_log2(uint64_t x) {
 // Average candidates per sub-bucket: large sub-buckets amortise each
 // workgroup's fixed work (bounds, table clear, fine-count scan); the cap is
 // kUniqueCap = 5120 and counts stay within a few percent of the mean.
-constexpr uint64_t kTargetBucket = 4800;
+constexpr uint64_t kTargetBucket = 5000;
 constexpr uint32_t kPhaseCount = 4;       // expand, partition, unique, compact
 constexpr uint32_t kLaneParts = 32;       // slabs per owner lane (cursor contention / P)
 
@@ -157,7 +157,7 @@ class SortedEngine final : public Engine {
       // half (measured on S13: partition 43-46 -> 39 ms, expand +2 ms).
       const uint32_t K1 = K <= kMaxPassBits ? K : std::min<uint32_t>(kMaxPassBits, (K + 1) / 2 + 1);
       const uint32_t K2 = K - K1;
-      uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 2048));
+      uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 8192));
       // Direct (K2 = 0): the lanes are the sub-buckets, one slab each.
       // Otherwise every lane part receives the children of NB1 / P blocks
       // (NB1 is a multiple of P), i.e. C / (lanes * P) on average.
