@@ -477,20 +477,27 @@ extern "C" int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_se
       s = e->advance(&nc, &f);
       ++done;
     }
+    e->settle();
     return s;
   })
 }
 
 extern "C" int mgbfs_engine_query(void* engine, mgbfs_engine_info* info) {
   if (!engine || !info) return hipErrorInvalidValue;
-  static_cast<Engine*>(engine)->fill_info(*info);
-  return 0;
+  MGBFS_GUARD({
+    static_cast<Engine*>(engine)->settle();
+    static_cast<Engine*>(engine)->fill_info(*info);
+    return 0;
+  })
 }
 
 extern "C" int mgbfs_engine_phases(void* engine, double* seconds, uint64_t* bytes, uint32_t count) {
   if (!engine || !seconds || !bytes) return hipErrorInvalidValue;
-  static_cast<Engine*>(engine)->fill_phases(seconds, bytes, count);
-  return 0;
+  MGBFS_GUARD({
+    static_cast<Engine*>(engine)->settle();
+    static_cast<Engine*>(engine)->fill_phases(seconds, bytes, count);
+    return 0;
+  })
 }
 
 extern "C" int mgbfs_engine_reset(void* engine) {

@@ -79,6 +79,9 @@ class Engine {
   virtual void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) = 0;
   virtual const char* name() const = 0;
   virtual uint64_t retry_count() const { return 0; }
+  // Waits for device work that the last advance left queued on the engine
+  // stream and folds its timing into the counters below.
+  virtual void settle() {}
   virtual void fill_info(mgbfs_engine_info& info) const;
   // Per-phase device seconds and modelled streaming bytes (backend-defined).
   virtual void fill_phases(double* seconds, uint64_t* bytes, uint32_t n) const {

@@ -58,8 +58,10 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K
                               uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
                               bool merge, uint32_t grid_limit, hipStream_t stream);
-hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, uint64_t* bank,
-                          hipStream_t stream);
+// Writes nothing unless *error == 0 and the layer total off[2^Ko] fits
+// bank_rows, so it may be queued before the host has read either.
+hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, const uint32_t* error,
+                          uint64_t bank_rows, uint64_t* bank, hipStream_t stream);
 
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);
 hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);

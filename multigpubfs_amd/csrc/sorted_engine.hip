@@ -58,6 +58,8 @@ class SortedEngine final : public Engine {
     MGBFS_HIP(hipSetDevice(device_));
     MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto& e : ev_) MGBFS_HIP(hipEventCreate(&e));
+    for (auto& p : cev_)
+      for (auto& e : p) MGBFS_HIP(hipEventCreate(&e));
     budget_ = vram_budget(o);
     cap_limit_ = o.capacity ? o.capacity : ~0ull;
     flags_ = o.flags;
@@ -94,6 +96,9 @@ class SortedEngine final : public Engine {
     if (host_) (void)hipHostFree(host_);
     for (auto& e : ev_)
       if (e) (void)hipEventDestroy(e);
+    for (auto& p : cev_)
+      for (auto& e : p)
+        if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -110,6 +115,7 @@ class SortedEngine final : public Engine {
                           stream_));
     MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 8, stream_));
     MGBFS_HIP(hipStreamSynchronize(stream_));
+    cpending_[0] = cpending_[1] = false;
     layers.assign(1, 1);
     candidates = unique_candidates = 0;
     kernel_seconds = 0.0;
@@ -196,7 +202,7 @@ class SortedEngine final : public Engine {
           probe_prev ? bank_[prev_].as<uint64_t>() : nullptr, probe_prev ? off_[prev_].as<uint32_t>() : nullptr,
           probe_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(), control, control + 2,
           layout_.tagbits, layout_.prune != 0, unique_grid_, stream_));
-      MGBFS_HIP(hipEventRecord(ev_[6], stream_));
+      MGBFS_HIP(hipEventRecord(ev_[4], stream_));
       MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), 1ull << Ko, off_[nxt].as<uint32_t>(),
                                    scan_ws_.as<uint32_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[3], stream_));
@@ -204,7 +210,15 @@ class SortedEngine final : public Engine {
       MGBFS_HIP(hipMemcpyAsync(host_, control_.as<uint64_t>(), 24, hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 3), off_[nxt].as<uint32_t>() + (1ull << Ko), 4,
                                hipMemcpyDeviceToHost, stream_));
-      MGBFS_HIP(hipStreamSynchronize(stream_));
+      MGBFS_HIP(hipEventRecord(ev_[5], stream_));
+      // The compaction is queued before the host has read N or the error
+      // word; it writes nothing unless the layer is clean and fits the bank
+      // as allocated now, and the host relaunches it after growing the bank.
+      // The host wakes on the readback alone, so its turnaround and the next
+      // layer's launches overlap the compaction.
+      const uint64_t early_rows = bank_cap_[nxt];
+      if (early_rows) compact(sub, K, Ko, nxt);
+      MGBFS_HIP(hipEventSynchronize(ev_[5]));
       const uint32_t err = static_cast<uint32_t>(host_[1]);
       const uint64_t C = host_[2];
       const uint64_t N = static_cast<uint32_t>(host_[3]);
@@ -239,14 +253,7 @@ class SortedEngine final : public Engine {
         terminal = true;
         return MGBFS_RESOURCE;
       }
-      MGBFS_HIP(hipEventRecord(ev_[4], stream_));
-      MGBFS_HIP(sorted_compact(sub, K, Ko, off_[nxt].as<uint32_t>(), bank_[nxt].as<uint64_t>(), stream_));
-      MGBFS_HIP(hipEventRecord(ev_[5], stream_));
-      MGBFS_HIP(hipEventSynchronize(ev_[5]));
-      float ms = 0.f;
-      MGBFS_HIP(hipEventElapsedTime(&ms, ev_[4], ev_[5]));
-      phase_seconds[3] += ms * 1e-3;
-      kernel_seconds += ms * 1e-3;
+      if (N > early_rows) compact(sub, K, Ko, nxt);
       phase_bytes[3] += 16 * N;
       streamed_bytes += 16 * N;
       launches += 1;
@@ -276,6 +283,10 @@ class SortedEngine final : public Engine {
   const char* name() const override { return "SORTED_THREE_BANK_LDS_UNIQUE"; }
   uint64_t retry_count() const override { return retries; }
 
+  void settle() override {
+    for (int s = 0; s < 2; ++s) read_compact(s);
+  }
+
   double phase_seconds[kPhaseCount] = {0, 0, 0, 0};
   uint64_t phase_bytes[kPhaseCount] = {0, 0, 0, 0};
   uint64_t retries = 0;
@@ -289,6 +300,29 @@ class SortedEngine final : public Engine {
 
  private:
   uint32_t* error_word() { return reinterpret_cast<uint32_t*>(control_.as<uint64_t>() + 1); }
+  // Compactions run unwaited; each is bracketed by one of two event pairs,
+  // read (and charged to phase 3) when the pair is reused or on settle().
+  // A pair is reused two compactions later, after the host has waited on
+  // the stream past it (a later readback, or the sync before a bank grows),
+  // so that read does not block.
+  void compact(uint64_t* sub, uint32_t K, uint32_t Ko, int nxt) {
+    const int s = cslot_ ^= 1;
+    read_compact(s);
+    MGBFS_HIP(hipEventRecord(cev_[s][0], stream_));
+    MGBFS_HIP(sorted_compact(sub, K, Ko, off_[nxt].as<uint32_t>(), error_word(), bank_cap_[nxt],
+                             bank_[nxt].as<uint64_t>(), stream_));
+    MGBFS_HIP(hipEventRecord(cev_[s][1], stream_));
+    cpending_[s] = true;
+  }
+  void read_compact(int s) {
+    if (!cpending_[s]) return;
+    MGBFS_HIP(hipEventSynchronize(cev_[s][1]));
+    float ms = 0.f;
+    MGBFS_HIP(hipEventElapsedTime(&ms, cev_[s][0], cev_[s][1]));
+    phase_seconds[3] += ms * 1e-3;
+    kernel_seconds += ms * 1e-3;
+    cpending_[s] = false;
+  }
   // Routing output size as passed to the kernels; the test flag reports an
   // oversized output so they take their 64-bit addressing path.
   uint64_t routed(uint64_t entries) const {
@@ -306,8 +340,8 @@ class SortedEngine final : public Engine {
     float a = 0, b = 0, c = 0, d = 0;
     MGBFS_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
     MGBFS_HIP(hipEventElapsedTime(&b, ev_[1], ev_[2]));
-    MGBFS_HIP(hipEventElapsedTime(&c, ev_[2], ev_[6]));
-    MGBFS_HIP(hipEventElapsedTime(&d, ev_[6], ev_[3]));
+    MGBFS_HIP(hipEventElapsedTime(&c, ev_[2], ev_[4]));
+    MGBFS_HIP(hipEventElapsedTime(&d, ev_[4], ev_[3]));
     phase_seconds[0] += a * 1e-3;
     phase_seconds[1] += b * 1e-3;
     phase_seconds[2] += c * 1e-3;
@@ -383,7 +417,9 @@ class SortedEngine final : public Engine {
   uint32_t ko_[3] = {0, 0, 0};
   int cur_ = 0, prev_ = -1;
   uint64_t* host_ = nullptr;
-  hipEvent_t ev_[7] = {};
+  hipEvent_t ev_[6] = {}, cev_[2][2] = {};
+  bool cpending_[2] = {false, false};
+  int cslot_ = 0;
 };
 
 Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o) { return new SortedEngine(g, o); }

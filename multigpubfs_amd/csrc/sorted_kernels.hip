@@ -564,10 +564,13 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
 }
 
 // grid = 2^K blocks: survivors of sub-bucket f (at the start of its slab)
-// -> bank rows.
+// -> bank rows.  A layer with an error bit (offsets not trustworthy) or with
+// more rows than the bank holds is left to the host.
 __global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand, uint32_t K,
                                                                 uint32_t Ko, const uint32_t* __restrict__ off,
-                                                                uint64_t* __restrict__ bank) {
+                                                                const uint32_t* __restrict__ error,
+                                                                uint64_t bank_rows, uint64_t* __restrict__ bank) {
+  if (*error != 0 || off[1ull << Ko] > bank_rows) return;
   const uint32_t f = blockIdx.x;
   const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
   const uint32_t sh = Ko - K;
@@ -662,9 +665,10 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K
   return hipGetLastError();
 }
 
-hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, uint64_t* bank,
-                          hipStream_t stream) {
-  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, K, Ko, off, bank);
+hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, const uint32_t* error,
+                          uint64_t bank_rows, uint64_t* bank, hipStream_t stream) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, K, Ko, off, error,
+                     bank_rows, bank);
   return hipGetLastError();
 }
 
