@@ -431,15 +431,15 @@ __device__ __forceinline__ void load_bank(const UniqueArgs& a, const BucketBound
   }
 }
 
-constexpr uint32_t kNoSlot = ~0u;
+constexpr int kPairsPerThread = kUniqueSlots / 2 / kUniqueThreads;  // table slot pairs per thread
 
 // Block x handles sub-buckets x, x + grid, ... (the engine launches one
 // block per sub-bucket).  The candidates and the first bank chunks are
 // loaded before the LDS table is built.  Phases: insert every candidate
-// (in-layer dedup; state equality ignores tags), each thread remembering
-// the slots it claimed -> probe both banks, marking matched slots -> the
-// claimed, unmarked slots are the survivors: count them per fine bucket,
-// scan, and write them back into the slab grouped by fine bucket.
+// (in-layer dedup; state equality ignores tags) -> probe the retained
+// banks, marking matched slots -> the occupied, unmarked slots are the
+// survivors: count them per fine bucket, scan, and write them back into
+// the slab grouped by fine bucket.
 __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void unique_diff_kernel(
     UniqueArgs a) {
   __shared__ UniqueLds u;
@@ -471,44 +471,66 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       u.unique = 0;
     }
     __syncthreads();
-    uint32_t slot_of[kUniquePerThread];
-    uint32_t inserted = 0;
-#pragma unroll
-    for (int k = 0; k < kUniquePerThread; ++k) {
-      slot_of[k] = kNoSlot;
-      if (s + k * kUniqueThreads + threadIdx.x >= e) continue;
-      const uint64_t h = dcur.v[k];
-      if (h == kEmpty) {
-        atomicOr(&u.flags, 1u);
-        continue;
-      }
-      const uint64_t key = h | a.tagmask;
+    // Each lane works through its own list of candidates, one CAS per round,
+    // moving to its next candidate as soon as the current one is placed, so
+    // a wave's rounds follow its busiest lane's total probe count instead of
+    // the sum of per-candidate maxima.  No stored value is all ones (the
+    // mark bit is clear in candidates and bank entries), so kEmpty is only
+    // ever an empty slot.
+    {
+      uint32_t left = c > threadIdx.x ? (static_cast<uint32_t>(c) - threadIdx.x + kUniqueThreads - 1) / kUniqueThreads : 0;
+      uint64_t h = dcur.v[0];
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
-      bool claimed = false;
-      while (true) {
+      while (left) {
         const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
-        if (old == kEmpty) {
-          claimed = true;
-          break;
-        }
-        if ((old | a.tagmask) == key) {
+        bool done = old == kEmpty;
+        if (!done && (old | a.tagmask) == (h | a.tagmask)) {
           if (a.merge) atomicOr(reinterpret_cast<unsigned long long*>(&u.table[slot]), h & a.tagmask);
-          break;
+          done = true;
         }
-        slot = (slot + 1) & mask;
-      }
-      if (claimed) {
-        slot_of[k] = slot;
-        ++inserted;
+        if (done) {
+          --left;
+#pragma unroll
+          for (int k = 0; k + 1 < kUniquePerThread; ++k) dcur.v[k] = dcur.v[k + 1];
+          h = dcur.v[0];
+          slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
+        } else {
+          slot = (slot + 1) & mask;
+        }
       }
     }
-    if (inserted) atomicAdd(&u.unique, inserted);
     __syncthreads();
-    // Difference against the two retained layers (same sub-bucket only).
+    // Difference against the retained layers (same sub-bucket only), the
+    // prefetched entries as one list per lane like the inserts; list
+    // positions past a bank's slice are skipped with one wasted read.
+    {
+      uint32_t live = 0;
 #pragma unroll
-    for (int k = 0; k < kBankPrefetch; ++k) {
-      uint64_t i;
-      if (bank_slot(a, bcur, k, &i)) probe_value(u, dcur.bpre[k], a, mask);
+      for (int k = 0; k < kBankPrefetch; ++k) {
+        uint64_t i;
+        if (bank_slot(a, bcur, k, &i)) live |= 1u << k;
+      }
+      uint32_t left = live ? 32 - __builtin_clz(live) : 0;
+      uint64_t h = dcur.bpre[0];
+      uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
+      while (left) {
+        const uint64_t v = u.table[slot];
+        bool done = !(live & 1u) || v == kEmpty;
+        if (!done && (v | a.tagmask) == (h | a.tagmask)) {
+          u.table[slot] = v | a.mark;
+          done = true;
+        }
+        if (done) {
+          --left;
+          live >>= 1;
+#pragma unroll
+          for (int k = 0; k + 1 < kBankPrefetch; ++k) dcur.bpre[k] = dcur.bpre[k + 1];
+          h = dcur.bpre[0];
+          slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
+        } else {
+          slot = (slot + 1) & mask;
+        }
+      }
     }
     const int ncur = cur_prefetch(a);
     probe_rest(u, a.cur, bcur.clo, bcur.chi, ncur, a, mask);
@@ -516,18 +538,31 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // The all-ones state cannot live in the table (it is the empty marker).
     const bool max_survives = (u.flags & 3u) == 1u;
-    // Survivors are the claimed, unmarked slots; the table entry carries the
-    // merged back-edge mask of all duplicates.
-    uint32_t survive = 0;
+    // Survivors are the occupied, unmarked slots; each entry carries the
+    // merged back-edge mask of its duplicates.  Thread t owns slot pairs
+    // t + j * kUniqueThreads (conflict-free 16-byte reads).
+    uint32_t survive = 0, occupied = 0;
 #pragma unroll
-    for (int k = 0; k < kUniquePerThread; ++k) {
-      if (slot_of[k] == kNoSlot) continue;
-      const uint64_t kept = u.table[slot_of[k]];
-      if (kept & a.mark) continue;
-      dcur.v[k] = kept;
-      survive |= 1u << k;
-      atomicAdd(&u.fhist[mid_bits(kept, a.K, fine_bits)], 1u);
+    for (int j = 0; j < kPairsPerThread; ++j) {
+      const uint32_t i = threadIdx.x + j * kUniqueThreads;
+      if (2 * i >= slots) continue;
+      const ulonglong2 p = table2[i];
+      if (p.x != kEmpty) {
+        ++occupied;
+        if (!(p.x & a.mark)) {
+          survive |= 1u << (2 * j);
+          atomicAdd(&u.fhist[mid_bits(p.x, a.K, fine_bits)], 1u);
+        }
+      }
+      if (p.y != kEmpty) {
+        ++occupied;
+        if (!(p.y & a.mark)) {
+          survive |= 2u << (2 * j);
+          atomicAdd(&u.fhist[mid_bits(p.y, a.K, fine_bits)], 1u);
+        }
+      }
     }
+    if (occupied) atomicAdd(&u.unique, occupied);
     if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
     __syncthreads();
     // One wave turns the fine counts into write cursors (exclusive scan by
@@ -548,10 +583,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         }
     }
     __syncthreads();
-    // Every survivor is in registers: write it back in place.
+    // Survivors are re-read from the table and written back in place.
 #pragma unroll
-    for (int k = 0; k < kUniquePerThread; ++k)
-      if ((survive >> k) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(dcur.v[k], a.K, fine_bits)], 1u)] = dcur.v[k];
+    for (int j = 0; j < kPairsPerThread; ++j) {
+      if (!((survive >> (2 * j)) & 3u)) continue;
+      const ulonglong2 p = table2[threadIdx.x + j * kUniqueThreads];
+      if ((survive >> (2 * j)) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(p.x, a.K, fine_bits)], 1u)] = p.x;
+      if ((survive >> (2 * j)) & 2u) a.cand[s + atomicAdd(&u.fhist[mid_bits(p.y, a.K, fine_bits)], 1u)] = p.y;
+    }
     if (threadIdx.x == 0 && max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
     if (threadIdx.x == 0) {
       unique_block += u.unique + (u.flags & 1u);
