@@ -300,8 +300,11 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
 
 // ------------------------------------------------------ unique + difference
 
+constexpr uint32_t kFilterWords = 2048;  // 64 Ki bits
+
 struct UniqueLds {
   alignas(16) uint64_t table[kUniqueSlots];  // read two slots at a time
+  alignas(16) uint32_t filter[kFilterWords];  // one bit per candidate state (key bits 13..28)
   uint32_t fhist[1u << kMaxFineBits];        // survivors, then write cursors, per fine bucket
   uint32_t flags;  // bit0: candidate ~0 present, bit1: ~0 found in a bank
   uint32_t unique;
@@ -431,6 +434,11 @@ __device__ __forceinline__ void load_bank(const UniqueArgs& a, const BucketBound
   }
 }
 
+// Filter bit of a stored value: key bits 13..28, above the table-slot bits.
+__device__ __forceinline__ uint32_t filter_bit(uint64_t h, uint32_t tagbits) {
+  return static_cast<uint32_t>((h >> tagbits) >> 13) & (kFilterWords * 32 - 1);
+}
+
 constexpr int kPairsPerThread = kUniqueSlots / 2 / kUniqueThreads;  // table slot pairs per thread
 
 // Block x handles sub-buckets x, x + grid, ... (the engine launches one
@@ -465,6 +473,8 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     const uint32_t mask = slots - 1;
     ulonglong2* table2 = reinterpret_cast<ulonglong2*>(u.table);
     for (uint32_t i = threadIdx.x; i < slots / 2; i += kUniqueThreads) table2[i] = make_ulonglong2(kEmpty, kEmpty);
+    for (uint32_t i = threadIdx.x; i < kFilterWords / 4; i += kUniqueThreads)
+      reinterpret_cast<uint4*>(u.filter)[i] = make_uint4(0, 0, 0, 0);
     for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
     if (threadIdx.x == 0) {
       u.flags = 0;
@@ -479,6 +489,13 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     // ever an empty slot.
     {
       uint32_t left = c > threadIdx.x ? (static_cast<uint32_t>(c) - threadIdx.x + kUniqueThreads - 1) / kUniqueThreads : 0;
+      // Every candidate's filter bit, before any bank entry tests it.
+#pragma unroll
+      for (int k = 0; k < kUniquePerThread; ++k)
+        if (k < static_cast<int>(left)) {
+          const uint32_t b = filter_bit(dcur.v[k], a.tagbits);
+          atomicOr(&u.filter[b >> 5], 1u << (b & 31));
+        }
       uint64_t h = dcur.v[0];
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
       while (left) {
@@ -502,20 +519,24 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // Difference against the retained layers (same sub-bucket only), the
     // prefetched entries as one list per lane like the inserts; list
-    // positions past a bank's slice are skipped with one wasted read.
+    // positions past a bank's slice, or whose filter bit is clear (no
+    // candidate has that state), are skipped without a table read.
     {
       uint32_t live = 0;
 #pragma unroll
       for (int k = 0; k < kBankPrefetch; ++k) {
         uint64_t i;
-        if (bank_slot(a, bcur, k, &i)) live |= 1u << k;
+        if (bank_slot(a, bcur, k, &i)) {
+          const uint32_t b = filter_bit(dcur.bpre[k], a.tagbits);
+          if ((u.filter[b >> 5] >> (b & 31)) & 1u) live |= 1u << k;
+        }
       }
       uint32_t left = live ? 32 - __builtin_clz(live) : 0;
       uint64_t h = dcur.bpre[0];
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
       while (left) {
-        const uint64_t v = u.table[slot];
-        bool done = !(live & 1u) || v == kEmpty;
+        const uint64_t v = (live & 1u) ? u.table[slot] : kEmpty;
+        bool done = v == kEmpty;
         if (!done && (v | a.tagmask) == (h | a.tagmask)) {
           u.table[slot] = v | a.mark;
           done = true;
