@@ -65,5 +65,7 @@ hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const u
 
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);
 hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
+hipError_t zero_u32x3(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb, uint32_t* c, uint64_t nc,
+                      hipStream_t stream);
 
 }  // namespace mgbfs

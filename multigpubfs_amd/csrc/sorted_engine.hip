@@ -181,9 +181,9 @@ class SortedEngine final : public Engine {
       ensure_u32(scount_, 1ull << K);
       ensure_u32(scan_ws_, scan_workspace_words(1ull << kMaxBucketBits) + 16);
       const uint32_t Ko = std::min<uint32_t>(kMaxBucketBits, K + margin_);
-      MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 3, stream_));
-      MGBFS_HIP(fill_u32(scount_.as<uint32_t>(), 0u, 1ull << K, stream_));
-      if (K2) MGBFS_HIP(fill_u32(lcount_.as<uint32_t>(), 0u, lslabs, stream_));
+      // Counters (unique, error, candidates), sub-bucket and lane cursors.
+      MGBFS_HIP(zero_u32x3(reinterpret_cast<uint32_t*>(control_.as<uint64_t>()), 6, scount_.as<uint32_t>(), 1ull << K,
+                           lcount_.as<uint32_t>(), K2 ? lslabs : 0, stream_));
       MGBFS_HIP(hipEventRecord(ev_[0], stream_));
       uint64_t* sub = sub_.as<uint64_t>();
       MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, K1, NB1, P, lcap,

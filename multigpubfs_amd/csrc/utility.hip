@@ -29,6 +29,26 @@ hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s
   return fill<uint32_t>(dst, value, count, stream);
 }
 
+// Three zero fills in one launch, over the concatenated index space.
+__global__ void zero3_kernel(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb, uint32_t* c, uint64_t nc) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+    if (i < na) a[i] = 0;
+    else if (i < na + nb) b[i - na] = 0;
+    else c[i - na - nb] = 0;
+  }
+}
+
+hipError_t zero_u32x3(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb, uint32_t* c, uint64_t nc,
+                      hipStream_t stream) {
+  const uint64_t count = na + nb + nc;
+  if (count == 0) return hipSuccess;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(zero3_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, a, na, b, nb, c, nc);
+  return hipGetLastError();
+}
+
 }  // namespace mgbfs
 
 extern "C" uint32_t mgbfs_hip_abi_version(void) { return MGBFS_HIP_ABI_VERSION; }
