@@ -40,27 +40,31 @@ hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank
 // area of kRouteDump entries (a slab overflow is written there; the layer
 // is then redone).
 constexpr uint64_t kRouteDump = 8192;
+// Sub-bucket of a stored value: floor(top32 * nsub / 2^32) (any nsub, a
+// multiple of 2^K2); lane = sub-bucket >> K2 (K2 = 0: lanes are sub-buckets).
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               bool masked, uint32_t nsub, uint32_t K2, uint32_t NB, uint32_t P, uint32_t cap,
+                               uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                uint64_t out_entries, hipStream_t stream);
 // Lane slabs (lcount per slab, capacity and stride lcap) -> sub-bucket
 // slabs of kUniqueCap entries, kSlabStride apart (scount per sub-bucket,
 // zeroed by the caller).
-hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t nsub,
                             uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, uint64_t out_entries,
                             hipStream_t stream);
 // In-layer dedup of the sub-bucket slabs and difference against the
-// current bank and, when given, the previous one; survivors are written back
-// grouped by Ko bits.  merge: OR duplicates' back-edge masks.
-hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
-                              const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
-                              uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
+// current bank and, when given, the previous one (banks partitioned into
+// nbo_c / nbo_p fine buckets); survivors are written back grouped by the
+// layer's nsub << fb fine buckets.  merge: OR duplicates' back-edge masks.
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t nsub, const uint64_t* cur,
+                              const uint32_t* cur_off, uint32_t nbo_c, const uint64_t* prev, const uint32_t* prev_off,
+                              uint32_t nbo_p, uint32_t fb, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
                               bool merge, uint32_t grid_limit, hipStream_t stream);
-// Writes nothing unless *error == 0 and the layer total off[2^Ko] fits
-// bank_rows, so it may be queued before the host has read either.
-hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, const uint32_t* error,
+// Writes nothing unless *error == 0 and the layer total off[nsub << fb]
+// fits bank_rows, so it may be queued before the host has read either.
+hipError_t sorted_compact(const uint64_t* cand, uint32_t nsub, uint32_t fb, const uint32_t* off, const uint32_t* error,
                           uint64_t bank_rows, uint64_t* bank, hipStream_t stream);
 
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);

@@ -110,7 +110,7 @@ class SortedEngine final : public Engine {
     last_ratio_ = 0.0;
     for (int b = 0; b < 3; ++b) count_[b] = 0;
     count_[0] = 1;
-    ko_[0] = kMaxBucketBits;
+    nbo_[0] = 1u << kMaxBucketBits;
     MGBFS_HIP(sorted_seed(start_key_, layout_, bank_[0].as<uint64_t>(), off_[0].as<uint32_t>(), kMaxBucketBits,
                           stream_));
     MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 8, stream_));
@@ -150,65 +150,75 @@ class SortedEngine final : public Engine {
     // costs one retry of the layer (slab or sub-bucket overflow).
     if (masked && last_ratio_ > 0)
       Cest = std::min<uint64_t>(Cest, static_cast<uint64_t>(static_cast<double>(F) * last_ratio_ * 1.1) + 4096);
-    uint32_t K = Cest <= kTargetBucket ? 0 : ceil_log2((Cest + kTargetBucket - 1) / kTargetBucket);
-    // Test hook: start three bits coarse so large layers exercise the
+    // Sub-buckets wanted: any count, not a power of two, so the mean fill
+    // stays near kTargetBucket on every layer.
+    const uint64_t kMaxSub = 1ull << kMaxBucketBits;
+    uint64_t want = std::min<uint64_t>(kMaxSub, Cest <= kTargetBucket ? 1 : (Cest + kTargetBucket - 1) / kTargetBucket);
+    // Test hook: start eight times coarse so large layers exercise the
     // bucket-overflow retry path.
-    if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && K >= 3) K -= 3;
+    if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && want >= 8) want >>= 3;
     const bool probe_prev = prev_ >= 0 && !layout_.prune;
-    uint32_t Klimit = std::min<uint32_t>(kMaxBucketBits, ko_[cur_]);
-    if (probe_prev) Klimit = std::min(Klimit, ko_[prev_]);
-    K = std::min(K, Klimit);
     while (true) {
-      // Two passes above kMaxPassBits: expansion takes one bit more than
-      // half (measured on S13: partition 43-46 -> 39 ms, expand +2 ms).
+      // K bits cover the sub-buckets.  Two passes above kMaxPassBits:
+      // expansion routes to lanes of 2^K2 consecutive sub-buckets, taking
+      // one bit more than half of K (measured on S13: partition 43-46 ->
+      // 39 ms, expand +2 ms); the sub-bucket count is rounded up to a whole
+      // number of lanes.
+      const uint32_t K = ceil_log2(want);
       const uint32_t K1 = K <= kMaxPassBits ? K : std::min<uint32_t>(kMaxPassBits, (K + 1) / 2 + 1);
       const uint32_t K2 = K - K1;
+      const uint32_t nsub = static_cast<uint32_t>(((want + (1ull << K2) - 1) >> K2) << K2);
+      const uint32_t lanes = nsub >> K2;
       uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 8192));
       // Direct (K2 = 0): the lanes are the sub-buckets, one slab each.
       // Otherwise every lane part receives the children of NB1 / P blocks
       // (NB1 is a multiple of P), i.e. C / (lanes * P) on average.
       const uint32_t P = K2 ? std::min<uint32_t>(kLaneParts, NB1) : 1;
       NB1 = (NB1 + P - 1) / P * P;
-      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(Cest) / ((1ull << K1) * P), slack_) : kUniqueCap;
-      const uint64_t lslabs = (1ull << K1) * P;
-      const uint64_t lane_entries = K2 ? lslabs * lcap : 0, sub_entries = (1ull << K) * kSlabStride;
+      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(Cest) / (static_cast<double>(lanes) * P), slack_)
+                               : kUniqueCap;
+      const uint64_t lslabs = static_cast<uint64_t>(lanes) * P;
+      const uint64_t lane_entries = K2 ? lslabs * lcap : 0, sub_entries = static_cast<uint64_t>(nsub) * kSlabStride;
       if (!ensure_slabs(lane_entries + kRouteDump, sub_entries + kRouteDump)) {
         *fatal = kErrFutureCap;
         terminal = true;
         return MGBFS_RESOURCE;
       }
       ensure_u32(lcount_, lslabs);
-      ensure_u32(scount_, 1ull << K);
+      ensure_u32(scount_, nsub);
       ensure_u32(scan_ws_, scan_workspace_words(1ull << kMaxBucketBits) + 16);
-      const uint32_t Ko = std::min<uint32_t>(kMaxBucketBits, K + margin_);
+      // The layer written is partitioned into nbo = nsub << fb fine buckets
+      // (at most 2^kMaxBucketBits), nested in the sub-buckets.
+      const uint32_t fb = std::min<uint32_t>(margin_, kMaxBucketBits - K);
+      const uint32_t nbo = nsub << fb;
       // Counters (unique, error, candidates), sub-bucket and lane cursors.
-      MGBFS_HIP(zero_u32x3(reinterpret_cast<uint32_t*>(control_.as<uint64_t>()), 6, scount_.as<uint32_t>(), 1ull << K,
+      MGBFS_HIP(zero_u32x3(reinterpret_cast<uint32_t*>(control_.as<uint64_t>()), 6, scount_.as<uint32_t>(), nsub,
                            lcount_.as<uint32_t>(), K2 ? lslabs : 0, stream_));
       MGBFS_HIP(hipEventRecord(ev_[0], stream_));
       uint64_t* sub = sub_.as<uint64_t>();
-      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, K1, NB1, P, lcap,
+      MGBFS_HIP(sorted_expand_route(bank_[cur_].as<uint64_t>(), F, gens_, layout_, masked, nsub, K2, NB1, P, lcap,
                                     K2 ? lcap : kSlabStride, K2 ? kErrLaneSlab : kErrBucketCap,
                                     K2 ? lcount_.as<uint32_t>() : scount_.as<uint32_t>(), error_word(),
                                     K2 ? lane_.as<uint64_t>() : sub, routed(K2 ? lane_entries : sub_entries),
                                     stream_));
       MGBFS_HIP(hipEventRecord(ev_[1], stream_));
       if (K2)
-        MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, K1, K2,
+        MGBFS_HIP(sorted_partition(lane_.as<uint64_t>(), lcount_.as<uint32_t>(), lcap, P, nsub, K2,
                                    scount_.as<uint32_t>(), error_word(), sub, routed(sub_entries), stream_));
       MGBFS_HIP(hipEventRecord(ev_[2], stream_));
       auto* control = reinterpret_cast<unsigned long long*>(control_.as<uint64_t>());
       MGBFS_HIP(sorted_unique_diff(
-          sub, scount_.as<uint32_t>(), K, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), ko_[cur_],
+          sub, scount_.as<uint32_t>(), nsub, bank_[cur_].as<uint64_t>(), off_[cur_].as<uint32_t>(), nbo_[cur_],
           probe_prev ? bank_[prev_].as<uint64_t>() : nullptr, probe_prev ? off_[prev_].as<uint32_t>() : nullptr,
-          probe_prev ? ko_[prev_] : 0, Ko, fine_counts_.as<uint32_t>(), error_word(), control, control + 2,
+          probe_prev ? nbo_[prev_] : 0, fb, fine_counts_.as<uint32_t>(), error_word(), control, control + 2,
           layout_.tagbits, layout_.prune != 0, unique_grid_, stream_));
       MGBFS_HIP(hipEventRecord(ev_[4], stream_));
-      MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), 1ull << Ko, off_[nxt].as<uint32_t>(),
+      MGBFS_HIP(exclusive_scan_u32(fine_counts_.as<uint32_t>(), nbo, off_[nxt].as<uint32_t>(),
                                    scan_ws_.as<uint32_t>(), stream_));
       MGBFS_HIP(hipEventRecord(ev_[3], stream_));
       // Readback: unique count, error word, candidate count, N (last offset).
       MGBFS_HIP(hipMemcpyAsync(host_, control_.as<uint64_t>(), 24, hipMemcpyDeviceToHost, stream_));
-      MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 3), off_[nxt].as<uint32_t>() + (1ull << Ko), 4,
+      MGBFS_HIP(hipMemcpyAsync(reinterpret_cast<uint32_t*>(host_ + 3), off_[nxt].as<uint32_t>() + nbo, 4,
                                hipMemcpyDeviceToHost, stream_));
       MGBFS_HIP(hipEventRecord(ev_[5], stream_));
       // The compaction is queued before the host has read N or the error
@@ -217,7 +227,7 @@ class SortedEngine final : public Engine {
       // The host wakes on the readback alone, so its turnaround and the next
       // layer's launches overlap the compaction.
       const uint64_t early_rows = bank_cap_[nxt];
-      if (early_rows) compact(sub, K, Ko, nxt);
+      if (early_rows) compact(sub, nsub, fb, nxt);
       MGBFS_HIP(hipEventSynchronize(ev_[5]));
       const uint32_t err = static_cast<uint32_t>(host_[1]);
       const uint64_t C = host_[2];
@@ -228,9 +238,9 @@ class SortedEngine final : public Engine {
         ++retries;
         continue;
       }
-      if (err & kErrBucketCap) {
-        if (K < Klimit) {
-          ++K;
+      if (err & kErrBucketCap) {  // a sub-bucket overflowed: twice as many
+        if (want < kMaxSub) {
+          want = std::min(kMaxSub, 2 * want);
           ++retries;
           continue;
         }
@@ -253,11 +263,11 @@ class SortedEngine final : public Engine {
         terminal = true;
         return MGBFS_RESOURCE;
       }
-      if (N > early_rows) compact(sub, K, Ko, nxt);
+      if (N > early_rows) compact(sub, nsub, fb, nxt);
       phase_bytes[3] += 16 * N;
       streamed_bytes += 16 * N;
       launches += 1;
-      ko_[nxt] = Ko;
+      nbo_[nxt] = nbo;
       count_[nxt] = N;
       prev_ = cur_;
       cur_ = nxt;
@@ -305,11 +315,11 @@ class SortedEngine final : public Engine {
   // A pair is reused two compactions later, after the host has waited on
   // the stream past it (a later readback, or the sync before a bank grows),
   // so that read does not block.
-  void compact(uint64_t* sub, uint32_t K, uint32_t Ko, int nxt) {
+  void compact(uint64_t* sub, uint32_t nsub, uint32_t fb, int nxt) {
     const int s = cslot_ ^= 1;
     read_compact(s);
     MGBFS_HIP(hipEventRecord(cev_[s][0], stream_));
-    MGBFS_HIP(sorted_compact(sub, K, Ko, off_[nxt].as<uint32_t>(), error_word(), bank_cap_[nxt],
+    MGBFS_HIP(sorted_compact(sub, nsub, fb, off_[nxt].as<uint32_t>(), error_word(), bank_cap_[nxt],
                              bank_[nxt].as<uint64_t>(), stream_));
     MGBFS_HIP(hipEventRecord(cev_[s][1], stream_));
     cpending_[s] = true;
@@ -414,7 +424,7 @@ class SortedEngine final : public Engine {
   uint64_t start_key_ = 0, budget_ = 0, cap_limit_ = 0, lane_cap_ = 0, sub_cap_ = 0;
   DeviceBuffer bank_[3], off_[3], lane_, sub_, lcount_, scount_, scan_ws_, fine_counts_, control_;
   uint64_t bank_cap_[3] = {0, 0, 0}, count_[3] = {0, 0, 0};
-  uint32_t ko_[3] = {0, 0, 0};
+  uint32_t nbo_[3] = {0, 0, 0};  // fine buckets of each bank
   int cur_ = 0, prev_ = -1;
   uint64_t* host_ = nullptr;
   hipEvent_t ev_[6] = {}, cev_[2][2] = {};

@@ -46,6 +46,13 @@ __device__ __forceinline__ uint32_t top_bits(uint64_t h, uint32_t bits) {
   return (static_cast<uint32_t>(h >> 32) >> 1) >> (31u - bits);
 }
 
+// Index of h among n equal ranges of its top 32 bits: floor(top32 * n / 2^32).
+// Monotone in h, so a bank ordered by its top bits is ordered by any such
+// index, and with n' = n << b every range of n' nests in one range of n.
+__device__ __forceinline__ uint32_t range_index(uint64_t h, uint32_t n) {
+  return __umulhi(static_cast<uint32_t>(h >> 32), n);
+}
+
 __device__ __forceinline__ uint32_t mid_bits(uint64_t h, uint32_t skip, uint32_t bits) {
   // bits [64-skip-bits, 64-skip) of h
   return bits ? static_cast<uint32_t>((h << skip) >> (64u - bits)) : 0u;
@@ -192,7 +199,8 @@ struct ExpandTile {
 template <int G, bool MASKED>
 __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uint64_t* __restrict__ bank, uint64_t F,
                                                                         PackedGenerators gens, SortedLayout L,
-                                                                        uint32_t K1, uint32_t NB, uint32_t P,
+                                                                        uint32_t nsub, uint32_t K2, uint32_t NB,
+                                                                        uint32_t P,
                                                                         uint32_t cap, uint32_t stride,
                                                                         uint32_t err_bit,
                                                                         uint32_t* __restrict__ cursors,
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
   using T = ExpandTile<G, MASKED>;
   static_assert(T::kRegs <= 32, "valid mask is 32 bits");
   __shared__ TileLds t;
-  const uint32_t buckets = 1u << K1;
+  const uint32_t buckets = nsub >> K2;  // lanes (sub-buckets when K2 = 0)
   const uint32_t part = blockIdx.x % P;
   for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
   __syncthreads();
@@ -210,7 +218,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     return reserve_run(&cursors[static_cast<uint64_t>(b) * P + part], cap, n, error, err_bit);
   };
   auto slab_base = [=](uint32_t b) { return (static_cast<uint64_t>(b) * P + part) * stride; };
-  auto bkt = [K1](uint64_t h) { return top_bits(h, K1); };
+  auto bkt = [nsub, K2](uint64_t h) { return range_index(h, nsub) >> K2; };
   // Tiles of contiguous parents go to blocks round-robin (tile i to block
   // i mod NB), so every lane part samples the whole frontier.
   const uint64_t tiles = (F + T::kParents - 1) / T::kParents;
@@ -267,7 +275,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 // sub-bucket slabs ((lane << K2) + b) * kSlabStride.
 __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const uint64_t* __restrict__ in,
                                                                            const uint32_t* __restrict__ lcount,
-                                                                           uint32_t lcap, uint32_t P, uint32_t K1,
+                                                                           uint32_t lcap, uint32_t P, uint32_t nsub,
                                                                            uint32_t K2,
                                                                            uint32_t* __restrict__ scount,
                                                                            uint32_t* __restrict__ error,
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
   __syncthreads();
   const uint64_t slab = blockIdx.x;
   const uint64_t r0 = slab * lcap, r1 = r0 + min(lcount[slab], lcap);
-  auto bkt = [K1, K2](uint64_t h) { return mid_bits(h, K1, K2); };
+  auto bkt = [nsub, K2](uint64_t h) { return range_index(h, nsub) & ((1u << K2) - 1u); };
   const uint64_t first = static_cast<uint64_t>(blockIdx.x / P) << K2;  // first sub-bucket of this lane
   auto reserve = [=](uint32_t b, uint32_t n) {
     return reserve_run(&scount[first + b], kUniqueCap, n, error, kErrBucketCap);
@@ -324,7 +332,7 @@ struct UniqueArgs {
   uint64_t tagmask;  // (1 << tagbits) - 1
   uint64_t mark;     // top tag bit, never set in a candidate: "present in a bank"
   uint32_t merge;    // OR the back-edge masks of duplicates into the kept entry
-  uint32_t K, Kc, Kp, Ko, tagbits, buckets;
+  uint32_t fb, nbo_c, nbo_p, tagbits, buckets;  // buckets: sub-buckets of this layer
 };
 
 constexpr uint64_t kEmpty = ~0ull;
@@ -382,13 +390,20 @@ __device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_
   if (f >= a.buckets) return b;
   b.s = static_cast<uint64_t>(f) * kSlabStride;
   b.e = b.s + a.scount[f];
+  // Sub-bucket f holds top-32 values [ceil(f 2^32 / n), ceil((f+1) 2^32 / n));
+  // a bank's slice is every fine bucket meeting that range (a boundary
+  // bucket may add states of a neighbour, which match nothing here).
+  const uint64_t n = a.buckets;
+  const uint32_t xlo = static_cast<uint32_t>(((static_cast<uint64_t>(f) << 32) + n - 1) / n);
+  const bool last = f + 1 == n;
+  const uint32_t xhi = last ? 0u : static_cast<uint32_t>(((static_cast<uint64_t>(f + 1) << 32) + n - 1) / n) - 1u;
   if (a.cur) {
-    b.clo = a.cur_off[static_cast<uint64_t>(f) << (a.Kc - a.K)];
-    b.chi = a.cur_off[static_cast<uint64_t>(f + 1) << (a.Kc - a.K)];
+    b.clo = a.cur_off[__umulhi(xlo, a.nbo_c)];
+    b.chi = a.cur_off[last ? a.nbo_c : __umulhi(xhi, a.nbo_c) + 1];
   }
   if (a.prev) {
-    b.plo = a.prev_off[static_cast<uint64_t>(f) << (a.Kp - a.K)];
-    b.phi = a.prev_off[static_cast<uint64_t>(f + 1) << (a.Kp - a.K)];
+    b.plo = a.prev_off[__umulhi(xlo, a.nbo_p)];
+    b.phi = a.prev_off[last ? a.nbo_p : __umulhi(xhi, a.nbo_p) + 1];
   }
   return b;
 }
@@ -451,7 +466,15 @@ constexpr int kPairsPerThread = kUniqueSlots / 2 / kUniqueThreads;  // table slo
 __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void unique_diff_kernel(
     UniqueArgs a) {
   __shared__ UniqueLds u;
-  const uint32_t fine_bits = a.Ko - a.K, fine = 1u << fine_bits;
+  const uint32_t fine_bits = a.fb, fine = 1u << fine_bits;
+  const uint32_t nbo = a.buckets << fine_bits;  // fine buckets of the layer written
+  // Fine bucket of a survivor of sub-bucket f, within f.  A layer whose
+  // lane slab overflowed holds stale entries of other sub-buckets (the
+  // layer is redone); clamping keeps their counter index in range.
+  auto fine_of = [nbo, fine_bits, fine](uint64_t h, uint32_t f) {
+    const uint32_t d = range_index(h, nbo) - (f << fine_bits);
+    return d < fine ? d : fine - 1u;
+  };
   unsigned long long unique_block = 0, cand_block = 0;
   for (uint32_t f = blockIdx.x; f < a.buckets; f += gridDim.x) {
     BucketData dcur;
@@ -572,14 +595,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         ++occupied;
         if (!(p.x & a.mark)) {
           survive |= 1u << (2 * j);
-          atomicAdd(&u.fhist[mid_bits(p.x, a.K, fine_bits)], 1u);
+          atomicAdd(&u.fhist[fine_of(p.x, f)], 1u);
         }
       }
       if (p.y != kEmpty) {
         ++occupied;
         if (!(p.y & a.mark)) {
           survive |= 2u << (2 * j);
-          atomicAdd(&u.fhist[mid_bits(p.y, a.K, fine_bits)], 1u);
+          atomicAdd(&u.fhist[fine_of(p.y, f)], 1u);
         }
       }
     }
@@ -609,8 +632,8 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     for (int j = 0; j < kPairsPerThread; ++j) {
       if (!((survive >> (2 * j)) & 3u)) continue;
       const ulonglong2 p = table2[threadIdx.x + j * kUniqueThreads];
-      if ((survive >> (2 * j)) & 1u) a.cand[s + atomicAdd(&u.fhist[mid_bits(p.x, a.K, fine_bits)], 1u)] = p.x;
-      if ((survive >> (2 * j)) & 2u) a.cand[s + atomicAdd(&u.fhist[mid_bits(p.y, a.K, fine_bits)], 1u)] = p.y;
+      if ((survive >> (2 * j)) & 1u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.x, f)], 1u)] = p.x;
+      if ((survive >> (2 * j)) & 2u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.y, f)], 1u)] = p.y;
     }
     if (threadIdx.x == 0 && max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
     if (threadIdx.x == 0) {
@@ -626,14 +649,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
 // grid = 2^K blocks: survivors of sub-bucket f (at the start of its slab)
 // -> bank rows.  A layer with an error bit (offsets not trustworthy) or with
 // more rows than the bank holds is left to the host.
-__global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand, uint32_t K,
-                                                                uint32_t Ko, const uint32_t* __restrict__ off,
+__global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* __restrict__ cand, uint32_t nsub,
+                                                                uint32_t fb, const uint32_t* __restrict__ off,
                                                                 const uint32_t* __restrict__ error,
                                                                 uint64_t bank_rows, uint64_t* __restrict__ bank) {
-  if (*error != 0 || off[1ull << Ko] > bank_rows) return;
+  if (*error != 0 || off[static_cast<uint64_t>(nsub) << fb] > bank_rows) return;
   const uint32_t f = blockIdx.x;
   const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
-  const uint32_t sh = Ko - K;
+  const uint32_t sh = fb;
   const uint64_t lo = off[static_cast<uint64_t>(f) << sh], hi = off[static_cast<uint64_t>(f + 1) << sh];
   for (uint64_t i = threadIdx.x; i < hi - lo; i += kRouteThreads) bank[lo + i] = cand[s + i];
 }
@@ -667,26 +690,29 @@ static uint32_t dump_index(uint64_t out_entries) {
 
 template <int G, bool MASKED>
 static hipError_t expand_route_g(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                                 uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                                 uint32_t nsub, uint32_t K2, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
                                  uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                  uint64_t out_entries, hipStream_t stream) {
   hipLaunchKernelGGL((expand_scatter_kernel<G, MASKED>), dim3(NB), dim3(kRouteThreads), 0, stream, bank, F, gens, L,
-                     K1, NB, P, cap, stride, err_bit, cursors, error, out, dump_index(out_entries));
+                     nsub, K2, NB, P, cap, stride, err_bit, cursors, error, out, dump_index(out_entries));
   return hipGetLastError();
 }
 
 hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGenerators& gens, const SortedLayout& L,
-                               bool masked, uint32_t K1, uint32_t NB, uint32_t P, uint32_t cap, uint32_t stride,
+                               bool masked, uint32_t nsub, uint32_t K2, uint32_t NB, uint32_t P, uint32_t cap,
+                               uint32_t stride,
                                uint32_t err_bit, uint32_t* cursors, uint32_t* error, uint64_t* out,
                                uint64_t out_entries, hipStream_t stream) {
-  if (K1 > kMaxPassBits || NB == 0 || P == 0 || cap == 0 || stride < cap) return hipErrorInvalidValue;
+  if (nsub == 0 || K2 > kMaxPassBits || (nsub & ((1u << K2) - 1u)) || (nsub >> K2) > (1u << kMaxPassBits) ||
+      NB == 0 || P == 0 || cap == 0 || stride < cap)
+    return hipErrorInvalidValue;
   switch (gens.count) {
 #define MGBFS_G(k)                                                                                               \
   case k:                                                                                                        \
     return masked                                                                                                \
-               ? expand_route_g<k, true>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
+               ? expand_route_g<k, true>(bank, F, gens, L, nsub, K2, NB, P, cap, stride, err_bit, cursors, error, out, \
                                          out_entries, stream)                                               \
-               : expand_route_g<k, false>(bank, F, gens, L, K1, NB, P, cap, stride, err_bit, cursors, error, out, \
+               : expand_route_g<k, false>(bank, F, gens, L, nsub, K2, NB, P, cap, stride, err_bit, cursors, error, out, \
                                           out_entries, stream);
     MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
     MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
@@ -696,22 +722,26 @@ hipError_t sorted_expand_route(const uint64_t* bank, uint64_t F, const PackedGen
   }
 }
 
-hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t K1,
+hipError_t sorted_partition(const uint64_t* in, const uint32_t* lcount, uint32_t lcap, uint32_t P, uint32_t nsub,
                             uint32_t K2, uint32_t* scount, uint32_t* error, uint64_t* out, uint64_t out_entries,
                             hipStream_t stream) {
-  if (K2 == 0 || K2 > kMaxPassBits || K1 + K2 > kMaxBucketBits || P == 0 || lcap == 0) return hipErrorInvalidValue;
-  const uint32_t blocks = (1u << K1) * P;
-  hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, lcount, lcap, P, K1,
+  if (K2 == 0 || K2 > kMaxPassBits || (nsub & ((1u << K2) - 1u)) || nsub == 0 || nsub > (1u << kMaxBucketBits) ||
+      (nsub >> K2) > (1u << kMaxPassBits) || P == 0 || lcap == 0)
+    return hipErrorInvalidValue;
+  const uint32_t blocks = (nsub >> K2) * P;
+  hipLaunchKernelGGL(partition_scatter_kernel, dim3(blocks), dim3(kRouteThreads), 0, stream, in, lcount, lcap, P, nsub,
                      K2, scount, error, out, dump_index(out_entries));
   return hipGetLastError();
 }
 
-hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K, const uint64_t* cur,
-                              const uint32_t* cur_off, uint32_t Kc, const uint64_t* prev, const uint32_t* prev_off,
-                              uint32_t Kp, uint32_t Ko, uint32_t* fine_counts, uint32_t* error,
+hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t nsub, const uint64_t* cur,
+                              const uint32_t* cur_off, uint32_t nbo_c, const uint64_t* prev, const uint32_t* prev_off,
+                              uint32_t nbo_p, uint32_t fb, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
                               bool merge, uint32_t grid_limit, hipStream_t stream) {
-  if (K > Ko || Ko - K > kMaxFineBits || (cur && K > Kc) || (prev && K > Kp) || grid_limit == 0)
+  if (nsub == 0 || fb > kMaxFineBits || (static_cast<uint64_t>(nsub) << fb) > (1ull << kMaxBucketBits) ||
+      (cur && (nbo_c == 0 || nbo_c > (1u << kMaxBucketBits))) ||
+      (prev && (nbo_p == 0 || nbo_p > (1u << kMaxBucketBits))) || grid_limit == 0)
     return hipErrorInvalidValue;
   if (tagbits < 4 || tagbits > 60) return hipErrorInvalidValue;
   const uint64_t tagmask = (1ull << tagbits) - 1;
@@ -719,15 +749,15 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t K
   // G <= tagbits - 2), so the mark bit tagbits-1 is free and a marked entry
   // never equals the all-ones empty marker.
   UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, cand_total, tagmask,
-               1ull << (tagbits - 1), merge ? 1u : 0u, K, Kc, Kp, Ko, tagbits, 1u << K};
-  const uint32_t grid = std::min<uint32_t>(1u << K, grid_limit);
+               1ull << (tagbits - 1), merge ? 1u : 0u, fb, nbo_c, nbo_p, tagbits, nsub};
+  const uint32_t grid = std::min<uint32_t>(nsub, grid_limit);
   hipLaunchKernelGGL(unique_diff_kernel, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t sorted_compact(const uint64_t* cand, uint32_t K, uint32_t Ko, const uint32_t* off, const uint32_t* error,
+hipError_t sorted_compact(const uint64_t* cand, uint32_t nsub, uint32_t fb, const uint32_t* off, const uint32_t* error,
                           uint64_t bank_rows, uint64_t* bank, hipStream_t stream) {
-  hipLaunchKernelGGL(compact_kernel, dim3(1u << K), dim3(kRouteThreads), 0, stream, cand, K, Ko, off, error,
+  hipLaunchKernelGGL(compact_kernel, dim3(nsub), dim3(kRouteThreads), 0, stream, cand, nsub, fb, off, error,
                      bank_rows, bank);
   return hipGetLastError();
 }
