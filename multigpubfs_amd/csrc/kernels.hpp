@@ -32,7 +32,7 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out, uin
                               hipStream_t stream);
 hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
                        hipStream_t stream);
-// Children of bank[0..F) routed by their top K1 bits into slabs: slab
+// Children of bank[0..F) routed by lane (sub-bucket >> K2) into slabs: slab
 // (lane * P + blockIdx % P) of `cap` entries, `stride` entries apart,
 // cursors[] counts per slab
 // (zeroed by the caller); a run beyond `cap` is dropped and sets err_bit.

@@ -1,25 +1,29 @@
 // Sort-unique backend with three rotating VRAM banks (inverse-closed sets).
 //
 // States are stored as h = (keymix(key) << tagbits) | tag (keymix is a
-// bijection, so equality of the high bits is full-key equality).  A bank
-// holds one BFS layer, partitioned by the top Ko bits of h ("fine
-// buckets"), with an offset table of 2^Ko + 1 entries.  One layer d -> d+1:
-//   expand_scatter    : children of bank[d], bucketed by the top K1 bits into
-//        owner lanes (LDS counting sort per 8192-element tile).  Each tile
+// bijection, so equality of the high bits is full-key equality).  Buckets
+// are ranges of the top 32 bits x: sub-bucket floor(x nsub / 2^32) of a
+// layer's nsub (any count, a multiple of 2^K2), fine bucket
+// floor(x nbo / 2^32) with nbo = nsub << fb nested in it.  A bank holds one
+// BFS layer grouped by fine bucket, with an offset table of nbo + 1
+// entries.  One layer d -> d+1:
+//   expand_scatter    : children of bank[d], bucketed into owner lanes of
+//        2^K2 consecutive sub-buckets (LDS counting sort per 8192-element
+//        tile).  Each tile
 //        reserves its run in a lane slab with one atomic per lane; every
 //        lane is split into P parts (block b uses part b % P) so no cursor
 //        sees more than 1/P of the tiles.                    8F + 8C
-//   partition_scatter : each lane part split by the next K2 bits into
-//        sub-bucket slabs of kUniqueCap entries (atomic run reservation).
+//   partition_scatter : each lane part split into its 2^K2 sub-bucket
+//        slabs of kUniqueCap entries (atomic run reservation).
 //                                                             16C
-//   unique_diff       : one workgroup per sub-bucket (K = K1+K2 bits): LDS
+//   unique_diff       : one workgroup per sub-bucket: LDS
 //        open-addressing set of the candidates (in-layer dedup), probe with
 //        the same sub-bucket of bank[d] and bank[d-1] (exact three-bank
-//        retention for inverse-closed generators), survivors regrouped by Ko
-//        fine bits and written back in place.             8C + 8F + 8F' + 8N
+//        retention for inverse-closed generators), survivors regrouped by
+//        fine bucket and written back in place.           8C + 8F + 8F' + 8N
 //   compact           : survivors -> bank[d+1].              16N
 // A run that would overflow its slab is dropped and flagged; the host
-// retries the layer with more slab room (lanes) or one more bucket bit
+// retries the layer with more slab room (lanes) or twice the sub-buckets
 // (sub-buckets).  Order inside a slab depends on atomic order; the layer
 // sets do not.
 // Replaces the reference's sorted-run owner (cuda/segmented_hash_batch.cu:
@@ -40,22 +44,11 @@ constexpr int kUniqueThreads = 512;
 constexpr int kUniquePerThread = kUniqueCap / kUniqueThreads;  // candidates held in registers
 constexpr int kPrefetch = 4;  // independent loads issued before dependent LDS work
 
-// Top `bits` (0..31) bits of h, from the high word with two 32-bit shifts
-// (no select for bits = 0).
-__device__ __forceinline__ uint32_t top_bits(uint64_t h, uint32_t bits) {
-  return (static_cast<uint32_t>(h >> 32) >> 1) >> (31u - bits);
-}
-
 // Index of h among n equal ranges of its top 32 bits: floor(top32 * n / 2^32).
 // Monotone in h, so a bank ordered by its top bits is ordered by any such
 // index, and with n' = n << b every range of n' nests in one range of n.
 __device__ __forceinline__ uint32_t range_index(uint64_t h, uint32_t n) {
   return __umulhi(static_cast<uint32_t>(h >> 32), n);
-}
-
-__device__ __forceinline__ uint32_t mid_bits(uint64_t h, uint32_t skip, uint32_t bits) {
-  // bits [64-skip-bits, 64-skip) of h
-  return bits ? static_cast<uint32_t>((h << skip) >> (64u - bits)) : 0u;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -661,10 +654,10 @@ __global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* 
   for (uint64_t i = threadIdx.x; i < hi - lo; i += kRouteThreads) bank[lo + i] = cand[s + i];
 }
 
-// Start layer: bank[0] = mix(start); offsets at Ko bits.
+// Start layer: bank[0] = mix(start); offsets over 2^Ko fine buckets.
 __global__ void sorted_seed_kernel(uint64_t start_key, SortedLayout L, uint64_t* bank, uint32_t* off, uint32_t Ko) {
   const uint64_t h = keymix(start_key, L.km) << L.tagbits;
-  const uint32_t b = top_bits(h, Ko);
+  const uint32_t b = range_index(h, 1u << Ko);
   const uint64_t n = (1ull << Ko) + 1;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
