@@ -307,7 +307,6 @@ struct UniqueLds {
   alignas(16) uint64_t table[kUniqueSlots];  // read two slots at a time
   alignas(16) uint32_t filter[kFilterWords];  // one bit per candidate state (key bits 13..28)
   uint32_t fhist[1u << kMaxFineBits];        // survivors, then write cursors, per fine bucket
-  uint32_t flags;  // bit0: candidate ~0 present, bit1: ~0 found in a bank
   uint32_t unique;
 };
 
@@ -335,10 +334,6 @@ constexpr uint64_t kEmpty = ~0ull;
 // states; two banks are disjoint layers), so a plain store suffices;
 // concurrent probes passing the slot see the same state bits either way.
 __device__ __forceinline__ void probe_value(UniqueLds& u, uint64_t h, const UniqueArgs& a, uint32_t mask) {
-  if (h == kEmpty) {
-    atomicOr(&u.flags, 2u);
-    return;
-  }
   const uint64_t key = h | a.tagmask;
   uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
   for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -492,10 +487,7 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     for (uint32_t i = threadIdx.x; i < kFilterWords / 4; i += kUniqueThreads)
       reinterpret_cast<uint4*>(u.filter)[i] = make_uint4(0, 0, 0, 0);
     for (uint32_t i = threadIdx.x; i < fine; i += kUniqueThreads) u.fhist[i] = 0;
-    if (threadIdx.x == 0) {
-      u.flags = 0;
-      u.unique = 0;
-    }
+    if (threadIdx.x == 0) u.unique = 0;
     __syncthreads();
     // Each lane works through its own list of candidates, one CAS per round,
     // moving to its next candidate as soon as the current one is placed, so
@@ -573,8 +565,6 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     probe_rest(u, a.cur, bcur.clo, bcur.chi, ncur, a, mask);
     if (a.prev) probe_rest(u, a.prev, bcur.plo, bcur.phi, kBankPrefetch - ncur, a, mask);
     __syncthreads();
-    // The all-ones state cannot live in the table (it is the empty marker).
-    const bool max_survives = (u.flags & 3u) == 1u;
     // Survivors are the occupied, unmarked slots; each entry carries the
     // merged back-edge mask of its duplicates.  Thread t owns slot pairs
     // t + j * kUniqueThreads (conflict-free 16-byte reads).
@@ -600,7 +590,6 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       }
     }
     if (occupied) atomicAdd(&u.unique, occupied);
-    if (threadIdx.x == 0 && max_survives) atomicAdd(&u.fhist[fine - 1], 1u);
     __syncthreads();
     // One wave turns the fine counts into write cursors (exclusive scan by
     // shuffles, <= 16 counters per lane) while the others wait at a single
@@ -628,9 +617,8 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       if ((survive >> (2 * j)) & 1u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.x, f)], 1u)] = p.x;
       if ((survive >> (2 * j)) & 2u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.y, f)], 1u)] = p.y;
     }
-    if (threadIdx.x == 0 && max_survives) a.cand[s + atomicAdd(&u.fhist[fine - 1], 1u)] = kEmpty;
     if (threadIdx.x == 0) {
-      unique_block += u.unique + (u.flags & 1u);
+      unique_block += u.unique;
       cand_block += c;
     }
     __syncthreads();  // LDS reuse by the next bucket
