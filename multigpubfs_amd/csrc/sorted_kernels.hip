@@ -227,7 +227,6 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       live |= (q < F ? 1u : 0u) << m;
       pv[m] = bank[q < F ? q : F - 1];
     }
-    uint64_t v[T::kRegs];
     uint32_t valid = 0;
     bool empty_mask = false;
     uint64_t key[T::kParentsPerThread];
@@ -246,18 +245,58 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     }
     if (MASKED && empty_mask) atomicOr(error, kErrDomain);
     // Generator-major: each shift group is read once and applied to all of
-    // this thread's parents.
+    // this thread's parents.  With back-edge masks the child's tag is the
+    // bit of g's inverse (child_tag[g] = inverse + 1): the generator
+    // leading back here.
+    if constexpr (MASKED && G == 3) {
+      // Two slots per parent (its mask leaves at most two children): slot 0
+      // takes the first unmasked child, slot 1 the second, so only those are
+      // mixed and routed (a third of the child slots fewer).
+      constexpr int PP = T::kParentsPerThread;
+      uint64_t w[2 * PP];   // child keys, then stored values
+      uint32_t wt[2 * PP];  // their tags
+      uint32_t wvalid = 0;
+      uint64_t child[PP];
+      const uint32_t tag0 = L.prune ? (1u << (gens.child_tag[0] - 1)) : 0u;
+      const uint32_t tag1 = L.prune ? (1u << (gens.child_tag[1] - 1)) : 0u;
+      const uint32_t tag2 = L.prune ? (1u << (gens.child_tag[2] - 1)) : 0u;
+      apply_groups_n(key, child, gens, 0);
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      uint64_t child[T::kParentsPerThread];
-      apply_groups_n(key, child, gens, g);
-      // With back-edge masks the child's tag is the bit of g's inverse
-      // (child_tag[g] = inverse + 1): the generator leading back here.
-      const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
+      for (int m = 0; m < PP; ++m) w[2 * m] = child[m];
+      apply_groups_n(key, child, gens, 1);
 #pragma unroll
-      for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
+      for (int m = 0; m < PP; ++m) {
+        const bool first = !((valid >> (3 * m)) & 1u);  // child 0 masked: child 1 takes slot 0
+        w[2 * m] = first ? child[m] : w[2 * m];
+        wt[2 * m] = first ? tag1 : tag0;
+        w[2 * m + 1] = child[m];
+      }
+      apply_groups_n(key, child, gens, 2);
+#pragma unroll
+      for (int m = 0; m < PP; ++m) {
+        const uint32_t ok = (valid >> (3 * m)) & 7u;
+        const bool only = ok == 4u, second = (ok & 4u) && !only;
+        w[2 * m] = only ? child[m] : w[2 * m];
+        wt[2 * m] = only ? tag2 : wt[2 * m];
+        w[2 * m + 1] = second ? child[m] : w[2 * m + 1];
+        wt[2 * m + 1] = second ? tag2 : tag1;
+        wvalid |= ((1u << __popc(ok)) - 1u) << (2 * m);  // <= 2 children (non-empty mask)
+      }
+#pragma unroll
+      for (int k = 0; k < 2 * PP; ++k) w[k] = (keymix(w[k], L.km) << L.tagbits) | wt[k];
+      route_regs(t, w, wvalid, buckets, bkt, reserve, slab_base, out, dump);
+    } else {
+      uint64_t v[T::kRegs];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        uint64_t child[T::kParentsPerThread];
+        apply_groups_n(key, child, gens, g);
+        const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
+#pragma unroll
+        for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
+      }
+      route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
     }
-    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
   }
 }
 
