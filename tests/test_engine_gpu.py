@@ -60,6 +60,19 @@ def test_s8_lrx_layer_sets_bit_exact(backend):
     assert sum(sizes) == 40320
 
 
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 7])
+def test_sorted_small_degrees_layer_sets_bit_exact(n):
+    # One sub-bucket per layer for most of these: the fine buckets, the
+    # start bank's 2^20 fine buckets and the slices of a bank whose
+    # partition is much finer than the layer's sub-buckets.
+    ref = O.packed_bfs(n, keep_keys=True)
+    with NativeEngine(lrx(n), backend="sorted") as e:
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+    assert sum(sizes) == math.factorial(n)
+
+
 @pytest.mark.parametrize("backend", BACKENDS)
 def test_s10_lrx_matches_reference_golden_table(backend):
     with NativeEngine(lrx(10), backend=backend) as e:
