@@ -2,8 +2,8 @@
 //
 // Layer loop (one host readback per layer, plus a retry when a slab
 // overflows): expand+route into owner-lane slabs -> partition into
-// sub-bucket slabs -> unique_diff -> scan of fine counts -> read N ->
-// compact into the next bank -> rotate.
+// sub-bucket slabs -> unique_diff -> scan of fine counts -> compact into
+// the next bank (queued before the host reads N) -> read N -> rotate.
 // Retention is exact for inverse-closed generator sets: every neighbour of
 // layer d lies in layer d-1, d or d+1 (graph_definition.py:84-92; retention
 // rule history_layers = 3, crates/mgbfs-runtime/src/generic_distributed_run.rs:40).
@@ -142,7 +142,7 @@ class SortedEngine final : public Engine {
     const int nxt = next_bank();
     // Every parent yields at most G children, at most G-1 when it carries a
     // back-edge mask (every state but the start does); the slabs and the
-    // bucket bits are sized for that bound.
+    // sub-bucket count are sized for that bound.
     const bool masked = layout_.prune && layers.size() > 1;
     uint64_t Cest = masked ? F * (G > 1 ? G - 1 : 1) : Cmax;
     // Masked layers skip every back-edge, so the previous layer's measured
