@@ -31,6 +31,11 @@ struct PackedGenerators {
   uint32_t degree;  // n
   uint8_t src[kMaxGenerators][kMaxDegree];  // child[i] = parent[src[g][i]]
   uint8_t child_tag[kMaxGenerators];        // tag written into a child made by g
+  // Back-edge mask of a child made by g: bit b set for every b whose
+  // generator is g's inverse (a repeated generator appears once per copy),
+  // so the OR over a state's discoverers is exactly its set of generators
+  // leading back to the previous layer.
+  uint16_t child_mask[kMaxGenerators];
   uint32_t tag_shift;                        // bitmap backend: tag bit position (64 = off)
   // The same permutation as shift groups: child nibble i takes parent nibble
   // i + delta for every destination nibble in the group's mask, so a
@@ -146,6 +151,23 @@ __host__ inline uint64_t apply_packed_host(uint64_t key, const uint8_t* src, uin
   uint64_t child = 0;
   for (uint32_t i = 0; i < n; ++i) child |= ((key >> (4u * src[i])) & 15u) << (4u * i);
   return child;
+}
+
+// child_mask[a] = { b : src[b] o src[a] = identity }.  Returns false when
+// some generator has no inverse in the set (the set is not inverse-closed).
+__host__ inline bool fill_back_edge_masks(PackedGenerators& pg) {
+  bool closed = true;
+  for (uint32_t a = 0; a < pg.count; ++a) {
+    uint32_t m = 0;
+    for (uint32_t b = 0; b < pg.count; ++b) {
+      bool inverse = true;
+      for (uint32_t i = 0; i < pg.degree && inverse; ++i) inverse = pg.src[b][pg.src[a][i]] == i;
+      if (inverse) m |= 1u << b;
+    }
+    pg.child_mask[a] = static_cast<uint16_t>(m);
+    closed &= m != 0;
+  }
+  return closed;
 }
 
 // Lexicographic rank (position 0 most significant) of a nibble-packed

@@ -123,6 +123,7 @@ class BitmapEngine final : public Engine {
     if (bitmap_bytes + 64 * 1024 > budget_) throw std::runtime_error("BITMAP_VRAM_CAPACITY");
     bitmap_.allocate(bitmap_bytes);
     control_.allocate(64);
+    stash_.allocate(kStashRows * 8);
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), 64, hipHostMallocDefault));
     cap_limit_ = o.capacity ? o.capacity : domain_;
     // Frontier buffers grow on demand up to what the budget admits.
@@ -145,7 +146,8 @@ class BitmapEngine final : public Engine {
     read_control();
     if (host_ctl_[1]) throw std::runtime_error("BITMAP_START_DOMAIN");
     count_ = 1;
-    prev_count_ = 0;
+    prev_count_ = stash_count_ = 0;
+    complete_ = false;
     layers.assign(1, 1);
     candidates = unique_candidates = 0;
     kernel_seconds = 0.0;
@@ -170,6 +172,12 @@ class BitmapEngine final : public Engine {
     const uint64_t want = std::min(gens_.count * count_, remaining);
     if (want > next_cap_) grow_next(std::min(cap_limit_, std::max(want, next_cap_ * 2)));
     MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 1, stream_));
+    // The expansion overwrites next_, which holds the previous layer: keep its
+    // first kStashRows rows, which a sample taken after a failed advance reads.
+    stash_count_ = std::min<uint64_t>(prev_count_, kStashRows);
+    if (stash_count_)
+      MGBFS_HIP(hipMemcpyAsync(stash_.as<uint64_t>(), next_.as<uint64_t>(), stash_count_ * 8,
+                               hipMemcpyDeviceToDevice, stream_));
     // Chunk the frontier only when the next buffer cannot hold the worst case.
     const uint64_t chunk = want <= next_cap_ ? count_ : std::max<uint64_t>(1, next_cap_ / gens_.count);
     MGBFS_HIP(hipEventRecord(ev0_, stream_));
@@ -209,6 +217,7 @@ class BitmapEngine final : public Engine {
     *next_count = produced_all;
     if (produced_all == 0) {
       terminal = true;
+      complete_ = true;
       return MGBFS_COMPLETE;
     }
     std::swap(front_, next_);
@@ -227,6 +236,10 @@ class BitmapEngine final : public Engine {
     std::vector<uint64_t> keys;
     if (which == 0) {
       copy_keys(front_, count_, limit, keys);
+    } else if (terminal && !complete_) {
+      // A failed advance wrote children over next_: the stash holds the
+      // previous layer's first rows.
+      copy_keys(stash_, stash_count_, limit, keys);
     } else if (prev_count_ && layers.size() >= 2) {
       // After a swap the previous layer stays intact in next_ until the next advance.
       copy_keys(next_, prev_count_, limit, keys);
@@ -275,13 +288,20 @@ class BitmapEngine final : public Engine {
       for (auto& k : out) k &= key_mask_;
     }
   }
-  void update_bytes() { device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes(); }
+  void update_bytes() {
+    device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes() + stash_.bytes();
+  }
 
+  // Rows of the previous layer kept across an advance (the states.json
+  // previous-layer sample holds at most 1000 rows).
+  static constexpr uint64_t kStashRows = 1000;
   PackedGenerators gens_;
   uint32_t n_;
   uint64_t domain_ = 0, budget_ = 0, cap_limit_ = 0;
-  DeviceBuffer bitmap_, front_, next_, control_;
-  uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, start_key_ = 0, key_mask_ = ~0ull;
+  DeviceBuffer bitmap_, front_, next_, control_, stash_;
+  uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, stash_count_ = 0, start_key_ = 0,
+           key_mask_ = ~0ull;
+  bool complete_ = false;
   uint64_t* host_ctl_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };

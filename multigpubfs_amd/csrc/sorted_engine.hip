@@ -67,14 +67,12 @@ class SortedEngine final : public Engine {
     layout_.km = make_keymix(4 * n_);
     layout_.tagbits = 64 - 4 * n_;
     layout_.tagmask = layout_.tagbits ? ((1ull << layout_.tagbits) - 1) : 0;
-    const std::vector<int> inv = inverse_index(gens_);
-    for (int x : inv)
-      if (x < 0) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
+    if (!fill_back_edge_masks(gens_)) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
     // Back-edge masks use tag bits 0..G-1; bit tagbits-1 is the in-bank
     // mark and bit tagbits-2 stays clear, so no entry is all ones.
     layout_.prune = gens_.count + 2 <= layout_.tagbits && !(flags_ & MGBFS_FLAG_TEST_NO_PRUNE);
-    for (uint32_t a = 0; a < gens_.count; ++a)
-      gens_.child_tag[a] = layout_.prune ? static_cast<uint8_t>(inv[a] + 1) : 0;
+    if (!layout_.prune)
+      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_mask[a] = 0;
     history_layers = layout_.prune ? 2 : 3;
     margin_ = std::min<uint32_t>(kMaxFineBits, 2 * (ceil_log2(gens_.count) + 1));
     start_key_ = g.start_key();
@@ -133,18 +131,18 @@ class SortedEngine final : public Engine {
     MGBFS_HIP(hipSetDevice(device_));
     const uint64_t F = count_[cur_];
     const uint32_t G = gens_.count;
-    const uint64_t Cmax = F * G;
+    // Every parent yields at most G children, at most G-1 when it carries a
+    // back-edge mask (every state but the start does); the slabs, the
+    // sub-bucket count and the 32-bit candidate bound use that bound.
+    const bool masked = layout_.prune && layers.size() > 1;
+    const uint64_t Cmax = masked ? F * (G > 1 ? G - 1 : 1) : F * G;
     if (Cmax > 0xfffffff0ull) {
       *fatal = kErrFutureCap;
       terminal = true;
       return MGBFS_RESOURCE;
     }
     const int nxt = next_bank();
-    // Every parent yields at most G children, at most G-1 when it carries a
-    // back-edge mask (every state but the start does); the slabs and the
-    // sub-bucket count are sized for that bound.
-    const bool masked = layout_.prune && layers.size() > 1;
-    uint64_t Cest = masked ? F * (G > 1 ? G - 1 : 1) : Cmax;
+    uint64_t Cest = Cmax;
     // Masked layers skip every back-edge, so the previous layer's measured
     // children per parent (plus 10%) predicts this one closely; a low guess
     // costs one retry of the layer (slab or sub-bucket overflow).

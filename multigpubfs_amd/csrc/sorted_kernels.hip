@@ -245,8 +245,8 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     }
     if (MASKED && empty_mask) atomicOr(error, kErrDomain);
     // Generator-major: each shift group is read once and applied to all of
-    // this thread's parents.  With back-edge masks the child's tag is the
-    // bit of g's inverse (child_tag[g] = inverse + 1): the generator
+    // this thread's parents.  With back-edge masks the child's tag is
+    // child_mask[g]: the bits of every copy of g's inverse, the generators
     // leading back here.
     if constexpr (MASKED && G == 3) {
       // Two slots per parent (its mask leaves at most two children): slot 0
@@ -257,9 +257,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       uint32_t wt[2 * PP];  // their tags
       uint32_t wvalid = 0;
       uint64_t child[PP];
-      const uint32_t tag0 = L.prune ? (1u << (gens.child_tag[0] - 1)) : 0u;
-      const uint32_t tag1 = L.prune ? (1u << (gens.child_tag[1] - 1)) : 0u;
-      const uint32_t tag2 = L.prune ? (1u << (gens.child_tag[2] - 1)) : 0u;
+      const uint32_t tag0 = gens.child_mask[0], tag1 = gens.child_mask[1], tag2 = gens.child_mask[2];
       apply_groups_n(key, child, gens, 0);
 #pragma unroll
       for (int m = 0; m < PP; ++m) w[2 * m] = child[m];
@@ -291,7 +289,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       for (int g = 0; g < G; ++g) {
         uint64_t child[T::kParentsPerThread];
         apply_groups_n(key, child, gens, g);
-        const uint64_t tag = L.prune ? (1ull << (gens.child_tag[g] - 1)) : 0ull;
+        const uint64_t tag = gens.child_mask[g];
 #pragma unroll
         for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
       }

@@ -30,3 +30,18 @@ def test_key_mixer_is_bijective_and_spreads_buckets(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "keymix ok" in out.stdout
+
+
+def test_back_edge_masks_cover_repeated_generators(tmp_path):
+    """Back-edge masks of the sorted backend (fill_back_edge_masks in
+    multigpubfs_amd/csrc/common.hpp): mask values for LRX with a repeated
+    generator, and a CPU model of the pruned layer loop against plain BFS."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("hipcc not installed")
+    exe = tmp_path / "backedge_check"
+    subprocess.run([hipcc, "-std=c++17", "-O1", "-I", str(ROOT / "multigpubfs_amd" / "csrc"),
+                    str(ROOT / "tests" / "native" / "backedge_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "backedge ok" in out.stdout

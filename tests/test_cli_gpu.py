@@ -55,11 +55,21 @@ def test_large_last_layer_sample_is_subset(tmp_path):
     assert states["previous_small"] == [] or ref["layers"][-2] < 1000
 
 
-def test_resource_stop_is_incomplete_with_prefix(tmp_path):
-    report = run_graph(lrx(8), tmp_path / "r", device=0, capacity=50, memory_backend="generic")
+@pytest.mark.parametrize("memory_backend", ["generic", "permutation_bitmap", "segmented"])
+def test_resource_stop_is_incomplete_with_prefix(tmp_path, memory_backend):
+    # capacity=50 stops LRX on S8 mid-run.  The bitmap backend writes the
+    # failed layer's first chunk over the buffer that held the previous
+    # layer; states.json must still hold the oracle's last two finished layers.
+    out = tmp_path / "r"
+    report = run_graph(lrx(8), out, device=0, capacity=50, memory_backend=memory_backend)
     assert report["status"] == "INCOMPLETE" and report["reason"].startswith("RESOURCE_")
-    ref = O.packed_bfs(8)["layers"]
-    assert report["layer_sizes"] == ref[:len(report["layer_sizes"])]
+    ref = O.packed_bfs(8, keep_keys=True)
+    sizes = report["layer_sizes"]
+    assert len(sizes) >= 3 and sizes == ref["layers"][:len(sizes)]
+    states = json.loads((out / "states.json").read_text())
+    depth = len(sizes) - 1
+    assert sorted(O.pack(r) for r in states["current"]) == sorted(ref["keys"][depth])
+    assert sorted(O.pack(r) for r in states["previous_small"]) == sorted(ref["keys"][depth - 1])
 
 
 def test_output_directory_must_not_exist(tmp_path):

@@ -90,6 +90,77 @@ def test_s11_lrx_matches_oracle(backend):
     assert info["total_states"] == math.factorial(11)
 
 
+S13_FLAGS = {0: "default", 8: "no back-edge masks (three banks probed)",
+             4 | 1: "tight lane slabs and coarse sub-buckets (both retry paths)"}
+
+
+@pytest.mark.parametrize("flags", list(S13_FLAGS))
+def test_s13_sorted_matches_reference_golden_table(flags):
+    # Full LRX on S13 (6,227,020,800 states, 79 layers, peak 369,741,101 at
+    # depth 54): the reference's own table (data/reference/lrx13-layers.json,
+    # tests/test_lrx_layers.py:44-48).
+    print(f"S13 sorted flags={flags} ({S13_FLAGS[flags]})", flush=True)
+    golden = GOLDEN["layers"]["13"]
+    with NativeEngine(lrx(13), backend="sorted", flags=flags) as e:
+        sizes, _ = run_all_layers(e)
+        info = e.info()
+    assert sizes == golden
+    assert len(sizes) == 79 and sum(sizes) == math.factorial(13)
+    assert max(sizes) == 369_741_101 and sizes.index(max(sizes)) == 54
+    assert info["total_states"] == math.factorial(13)
+    assert info["history_layers"] == (3 if flags & 8 else 2)
+    if flags & 8:
+        assert info["candidates"] == 3 * math.factorial(13)
+    else:
+        # Every state but the start skips at least its discovering edge, and
+        # every state but the start is some parent's candidate.
+        assert math.factorial(13) - 1 <= info["candidates"] <= 2 * math.factorial(13) + 1
+    if flags == 4 | 1:
+        assert info["retries"] > 0
+
+
+def test_s13_bitmap_matches_reference_golden_table():
+    print("S13 permutation_bitmap", flush=True)
+    with NativeEngine(lrx(13), backend="permutation_bitmap") as e:
+        sizes, _ = run_all_layers(e)
+    assert sizes == GOLDEN["layers"]["13"]
+
+
+def lrx_variants(n):
+    L, R, X = O.lrx_generators(n)
+    return {"LRXX": [L, R, X, X], "LRXL": [L, R, X, L], "XLRX": [X, L, R, X], "LRLRX": [L, R, L, R, X]}
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("name", ["LRXX", "LRXL", "XLRX", "LRLRX"])
+def test_repeated_generators_match_oracle(backend, name):
+    # A generator given more than once: a discovered state must skip every
+    # copy of the generator leading back to its parent, not only the first.
+    n = 8
+    gens = lrx_variants(n)[name]
+    ref = O.packed_bfs(n, gens, keep_keys=True)
+    assert ref["layers"] == O.packed_bfs(n)["layers"]
+    with NativeEngine(GraphDefinition.permutation(gens, list(range(n))), backend=backend) as e:
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_sorted_random_sets_with_repeats_match_oracle(seed):
+    rng = random.Random(seed)
+    n = rng.choice([7, 8, 9])
+    gens = random_generators(rng, n, rng.randint(1, 3), True)
+    gens += [list(rng.choice(gens)) for _ in range(rng.randint(1, 2))]
+    rng.shuffle(gens)
+    ref = O.packed_bfs(n, gens, keep_keys=True)
+    with NativeEngine(GraphDefinition.permutation(gens, list(range(n))), backend="sorted") as e:
+        assert e.backend_name == NAMES["sorted"]
+        sizes, keys = run_all_layers(e, keep_keys=True)
+    assert sizes == ref["layers"]
+    assert keys == [sorted(k) for k in ref["keys"]]
+
+
 def test_s11_sorted_overflow_retry_path_is_exact():
     # Coarse start buckets force sub-bucket overflows and finer retries.
     ref = O.packed_bfs(11)["layers"]
