@@ -154,6 +154,79 @@ int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint3
                                uint64_t* next, uint64_t* next_count, uint64_t next_capacity, uint32_t* error,
                                void* stream);
 
+/* ------------------------------------------------ sorted-bank layers */
+/* The sort-unique owner as one call per BFS layer, for inverse-closed
+ * permutation generators on nibble-packed states (degree 1..15, at most 16
+ * generators).  Replaces the reference's sorted-run owner
+ * mgbfs_hash_batch_accept_timed (cuda/segmented_hash_batch.cu:157-215) and
+ * its batch owner route (cuda/segmented_batch_route.cu:52-151): children of
+ * a bank are routed into sub-buckets, deduplicated in LDS, differenced
+ * against the bank (and, without back-edge masks, the bank before it; the
+ * three-bank rule of crates/mgbfs-runtime/src/generic_distributed_run.rs:40)
+ * and compacted into the next bank.
+ *
+ * Bank format: one uint64 per state, (mix(key) << (64 - 4*degree)) | tag,
+ * where mix is a bijection on 4*degree bits and the tag holds the state's
+ * back-edge mask; a bank is grouped by `buckets` fine buckets with an offset
+ * table of buckets + 1 uint32 entries (offsets[buckets] = states).  Offset
+ * tables must hold MGBFS_SORTED_OFFSETS entries.  The seed bank has
+ * MGBFS_SORTED_SEED_BUCKETS fine buckets; a layer call writes
+ * plan.next_buckets.  mgbfs_sorted_unpack turns bank values into packed keys.
+ *
+ * After a layer call the next layer holds next_offsets[next_buckets] states
+ * (a device word).  Error bits, after which `next` is untouched and the
+ * layer is repeated with the error word and counters zeroed again:
+ *   64  a sub-bucket above the LDS capacity: pass sub_buckets = twice the
+ *       plan's sub_buckets;
+ *   256 an owner-lane slab overflowed: pass slack = twice the slack;
+ *   4   next_capacity below the layer: grow `next` to
+ *       next_offsets[next_buckets] rows. */
+#define MGBFS_SORTED_OFFSETS 1048577u     /* 2^20 + 1 */
+#define MGBFS_SORTED_SEED_BUCKETS 1048576u /* 2^20 */
+
+typedef struct mgbfs_sorted_plan_info {
+  uint64_t workspace_bytes;  /* device workspace mgbfs_sorted_layer needs        */
+  uint64_t candidates_bound; /* children generated at most                       */
+  uint32_t sub_buckets;      /* sub-buckets of the pass                           */
+  uint32_t next_buckets;     /* fine buckets of the bank written                  */
+  uint32_t masked;           /* 1: back-edge masks, `previous` is not read        */
+  uint32_t history_banks;    /* banks the caller rotates: 2 with masks, 3 without */
+} mgbfs_sorted_plan_info;
+
+typedef struct mgbfs_sorted_layer_args {
+  uint32_t degree, generators;
+  const uint32_t* permutations; /* HOST table [generators][degree]               */
+  uint32_t depth;               /* depth of `bank` (0 = the seed bank)           */
+  uint32_t sub_buckets;         /* 0 = sized from count                          */
+  uint32_t slack;               /* lane-slab slack, 0 or 1 = default             */
+  uint32_t buckets;             /* fine buckets of `bank`                        */
+  const uint64_t* bank;         /* layer `depth`, `count` states                 */
+  const uint32_t* offsets;
+  uint64_t count;
+  const uint64_t* previous;     /* layer depth-1 (read only without masks)       */
+  const uint32_t* previous_offsets;
+  uint32_t previous_buckets, reserved;
+  uint64_t* next;               /* receives layer depth+1                        */
+  uint32_t* next_offsets;
+  uint64_t next_capacity;       /* rows `next` holds                             */
+  uint64_t* counters;           /* device [2], caller-zeroed: += distinct
+                                   children, += children generated               */
+  uint32_t* error;              /* device sticky word, caller-zeroed             */
+} mgbfs_sorted_layer_args;
+
+/* Host-only: workspace size and geometry of one layer call. */
+int mgbfs_sorted_plan(uint32_t degree, uint32_t generators, const uint32_t* permutations, uint32_t depth,
+                      uint64_t count, uint32_t sub_buckets, uint32_t slack, mgbfs_sorted_plan_info* plan);
+/* Writes the one-state seed bank of `start_key` (MGBFS_SORTED_SEED_BUCKETS
+ * fine buckets). */
+int mgbfs_sorted_seed(uint32_t degree, uint64_t start_key, uint64_t* bank, uint32_t* offsets, void* stream);
+/* Enqueues one layer; `workspace` (256-byte aligned) holds at least
+ * plan.workspace_bytes.  Calls sharing a workspace must not overlap. */
+int mgbfs_sorted_layer(const mgbfs_sorted_layer_args* args, void* workspace, uint64_t workspace_bytes,
+                       void* stream);
+/* Host-only: bank values (host memory) -> nibble-packed keys. */
+int mgbfs_sorted_unpack(uint32_t degree, const uint64_t* values, uint64_t count, uint64_t* keys);
+
 /* ------------------------------------------------- batched instances */
 /* Many independent BFS instances advanced together, one layer of every
  * active instance per step, by a hipGraph-captured layer loop; instances are

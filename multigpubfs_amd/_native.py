@@ -240,6 +240,39 @@ class BatchOptions(C.Structure):
                 ("layers_per_graph", C.c_uint32), ("grid", C.c_uint32)]
 
 
+class SortedPlanInfo(C.Structure):
+    _fields_ = [("workspace_bytes", C.c_uint64), ("candidates_bound", C.c_uint64), ("sub_buckets", C.c_uint32),
+                ("next_buckets", C.c_uint32), ("masked", C.c_uint32), ("history_banks", C.c_uint32)]
+
+
+class SortedLayerArgs(C.Structure):
+    _fields_ = [("degree", C.c_uint32), ("generators", C.c_uint32), ("permutations", C.POINTER(C.c_uint32)),
+                ("depth", C.c_uint32), ("sub_buckets", C.c_uint32), ("slack", C.c_uint32), ("buckets", C.c_uint32),
+                ("bank", C.c_void_p), ("offsets", C.c_void_p), ("count", C.c_uint64),
+                ("previous", C.c_void_p), ("previous_offsets", C.c_void_p), ("previous_buckets", C.c_uint32),
+                ("reserved", C.c_uint32), ("next", C.c_void_p), ("next_offsets", C.c_void_p),
+                ("next_capacity", C.c_uint64), ("counters", C.c_void_p), ("error", C.c_void_p)]
+
+
+SORTED_OFFSETS = (1 << 20) + 1
+SORTED_SEED_BUCKETS = 1 << 20
+
+
+def bind_sorted(lib):
+    """Signatures of the kernel-level sorted-bank layer ABI (mgbfs_sorted_*)."""
+    if getattr(lib, "_sorted_bound", False):
+        return lib
+    p, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+    lib.mgbfs_sorted_plan.argtypes = [u32, u32, C.POINTER(u32), u32, u64, u32, u32, C.POINTER(SortedPlanInfo)]
+    lib.mgbfs_sorted_seed.argtypes = [u32, u64, p, p, p]
+    lib.mgbfs_sorted_layer.argtypes = [C.POINTER(SortedLayerArgs), p, u64, p]
+    lib.mgbfs_sorted_unpack.argtypes = [u32, C.POINTER(u64), u64, C.POINTER(u64)]
+    for name in ("plan", "seed", "layer", "unpack"):
+        getattr(lib, "mgbfs_sorted_" + name).restype = C.c_int
+    lib._sorted_bound = True
+    return lib
+
+
 MGBFS_BATCH_FULL = 3
 VISITED = {"auto": 0, "bitmap": 1, "hash": 2}
 

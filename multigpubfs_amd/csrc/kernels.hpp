@@ -67,6 +67,10 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t n
 hipError_t sorted_compact(const uint64_t* cand, uint32_t nsub, uint32_t fb, const uint32_t* off, const uint32_t* error,
                           uint64_t bank_rows, uint64_t* bank, hipStream_t stream);
 
+// Sets kErrFutureCap in *error when off[n] > rows.
+hipError_t sorted_capacity_check(const uint32_t* off, uint64_t n, uint64_t rows, uint32_t* error,
+                                 hipStream_t stream);
+
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t count, hipStream_t stream);
 hipError_t fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 hipError_t zero_u32x3(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb, uint32_t* c, uint64_t nc,

@@ -17,34 +17,13 @@
 #include <cstring>
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "sorted_plan.hpp"
 
 namespace mgbfs {
 
 namespace {
 
-uint32_t ceil_log2(uint64_t x) {
-  uint32_t b = 0;
-  while ((1ull << b) < x) ++b;
-  return b;
-}
-
-// Average candidates per sub-bucket: large sub-buckets amortise each
-// workgroup's fixed work (bounds, table clear, fine-count scan); the cap is
-// kUniqueCap = 5120 and counts stay within a few percent of the mean.
-constexpr uint64_t kTargetBucket = 5000;
-constexpr uint32_t kPhaseCount = 4;       // expand, partition, unique, compact
-constexpr uint32_t kLaneParts = 32;       // slabs per owner lane (cursor contention / P)
-
-// Slab capacity for `mean` expected entries: mean + slack * (mean / 4 +
-// 16 sqrt(mean) + 1024), rounded to 64 entries.  A lane part's children come
-// from contiguous tiles of a frontier ordered by mixed-key bits, and the
-// mixer's top bits are linear in the key, so lane counts spread wider than
-// a binomial; slack doubles after an overflow.
-uint32_t slab_capacity(double mean, uint32_t slack) {
-  const double c = mean + slack * (mean / 4.0 + 16.0 * std::sqrt(mean) + 1024.0);
-  const uint64_t r = (static_cast<uint64_t>(c) + 63) / 64 * 64;
-  return static_cast<uint32_t>(std::min<uint64_t>(r, 0xffffffc0ull));
-}
+constexpr uint32_t kPhaseCount = 4;  // expand, partition, unique, compact
 
 }  // namespace
 
@@ -74,7 +53,7 @@ class SortedEngine final : public Engine {
     if (!layout_.prune)
       for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_mask[a] = 0;
     history_layers = layout_.prune ? 2 : 3;
-    margin_ = std::min<uint32_t>(kMaxFineBits, 2 * (ceil_log2(gens_.count) + 1));
+    margin_ = fine_margin(gens_.count);
     start_key_ = g.start_key();
     // One workgroup per sub-bucket: persistent grids of 2 or 8 workgroups
     // per CU, prefetching the next sub-bucket's bounds, measured slower
@@ -148,35 +127,16 @@ class SortedEngine final : public Engine {
     // costs one retry of the layer (slab or sub-bucket overflow).
     if (masked && last_ratio_ > 0)
       Cest = std::min<uint64_t>(Cest, static_cast<uint64_t>(static_cast<double>(F) * last_ratio_ * 1.1) + 4096);
-    // Sub-buckets wanted: any count, not a power of two, so the mean fill
-    // stays near kTargetBucket on every layer.
     const uint64_t kMaxSub = 1ull << kMaxBucketBits;
-    uint64_t want = std::min<uint64_t>(kMaxSub, Cest <= kTargetBucket ? 1 : (Cest + kTargetBucket - 1) / kTargetBucket);
+    uint64_t want = sub_buckets_for(Cest);
     // Test hook: start eight times coarse so large layers exercise the
     // bucket-overflow retry path.
     if ((flags_ & MGBFS_FLAG_TEST_COARSE_BUCKETS) && want >= 8) want >>= 3;
     const bool probe_prev = prev_ >= 0 && !layout_.prune;
     while (true) {
-      // K bits cover the sub-buckets.  Two passes above kMaxPassBits:
-      // expansion routes to lanes of 2^K2 consecutive sub-buckets, taking
-      // one bit more than half of K (measured on S13: partition 43-46 ->
-      // 39 ms, expand +2 ms); the sub-bucket count is rounded up to a whole
-      // number of lanes.
-      const uint32_t K = ceil_log2(want);
-      const uint32_t K1 = K <= kMaxPassBits ? K : std::min<uint32_t>(kMaxPassBits, (K + 1) / 2 + 1);
-      const uint32_t K2 = K - K1;
-      const uint32_t nsub = static_cast<uint32_t>(((want + (1ull << K2) - 1) >> K2) << K2);
-      const uint32_t lanes = nsub >> K2;
-      uint32_t NB1 = static_cast<uint32_t>(std::clamp<uint64_t>((F + 2047) / 2048, 1, 8192));
-      // Direct (K2 = 0): the lanes are the sub-buckets, one slab each.
-      // Otherwise every lane part receives the children of NB1 / P blocks
-      // (NB1 is a multiple of P), i.e. C / (lanes * P) on average.
-      const uint32_t P = K2 ? std::min<uint32_t>(kLaneParts, NB1) : 1;
-      NB1 = (NB1 + P - 1) / P * P;
-      const uint32_t lcap = K2 ? slab_capacity(static_cast<double>(Cest) / (static_cast<double>(lanes) * P), slack_)
-                               : kUniqueCap;
-      const uint64_t lslabs = static_cast<uint64_t>(lanes) * P;
-      const uint64_t lane_entries = K2 ? lslabs * lcap : 0, sub_entries = static_cast<uint64_t>(nsub) * kSlabStride;
+      const SortedGeometry geo = sorted_geometry(want, F, Cest, slack_, margin_);
+      const uint32_t K2 = geo.K2, nsub = geo.nsub, NB1 = geo.NB1, P = geo.P, lcap = geo.lcap;
+      const uint64_t lslabs = geo.lslabs, lane_entries = geo.lane_entries, sub_entries = geo.sub_entries;
       if (!ensure_slabs(lane_entries + kRouteDump, sub_entries + kRouteDump)) {
         *fatal = kErrFutureCap;
         terminal = true;
@@ -185,10 +145,7 @@ class SortedEngine final : public Engine {
       ensure_u32(lcount_, lslabs);
       ensure_u32(scount_, nsub);
       ensure_u32(scan_ws_, scan_workspace_words(1ull << kMaxBucketBits) + 16);
-      // The layer written is partitioned into nbo = nsub << fb fine buckets
-      // (at most 2^kMaxBucketBits), nested in the sub-buckets.
-      const uint32_t fb = std::min<uint32_t>(margin_, kMaxBucketBits - K);
-      const uint32_t nbo = nsub << fb;
+      const uint32_t fb = geo.fb, nbo = geo.nbo;
       // Counters (unique, error, candidates), sub-bucket and lane cursors.
       MGBFS_HIP(zero_u32x3(reinterpret_cast<uint32_t*>(control_.as<uint64_t>()), 6, scount_.as<uint32_t>(), nsub,
                            lcount_.as<uint32_t>(), K2 ? lslabs : 0, stream_));

@@ -690,7 +690,19 @@ __global__ void sorted_seed_kernel(uint64_t start_key, SortedLayout L, uint64_t*
   if (blockIdx.x == 0 && threadIdx.x == 0) bank[0] = h;
 }
 
+// Sets kErrFutureCap when the layer total off[n] exceeds `rows` (the
+// compaction then writes nothing).
+__global__ void capacity_check_kernel(const uint32_t* off, uint64_t n, uint64_t rows, uint32_t* error) {
+  if (off[n] > rows) atomicOr(error, kErrFutureCap);
+}
+
 // ------------------------------------------------------------ launchers
+
+hipError_t sorted_capacity_check(const uint32_t* off, uint64_t n, uint64_t rows, uint32_t* error,
+                                 hipStream_t stream) {
+  hipLaunchKernelGGL(capacity_check_kernel, dim3(1), dim3(1), 0, stream, off, n, rows, error);
+  return hipGetLastError();
+}
 
 hipError_t sorted_seed(uint64_t start_key, const SortedLayout& L, uint64_t* bank, uint32_t* off, uint32_t Ko,
                        hipStream_t stream) {
