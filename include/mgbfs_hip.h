@@ -224,6 +224,17 @@ int mgbfs_sorted_seed(uint32_t degree, uint64_t start_key, uint64_t* bank, uint3
  * plan.workspace_bytes.  Calls sharing a workspace must not overlap. */
 int mgbfs_sorted_layer(const mgbfs_sorted_layer_args* args, void* workspace, uint64_t workspace_bytes,
                        void* stream);
+/* mgbfs_sorted_layer without a workspace argument, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The workspace is a
+ * static device array of MGBFS_SORTED_SCRATCH_BYTES per GPU, set up on the
+ * first call on each device and held until the process ends; the call
+ * allocates nothing.  Layers whose plan.workspace_bytes exceed it are
+ * rejected with hipErrorOutOfMemory (about 5 million children per layer
+ * fit).  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (different streams or threads) race, so serialise them. */
+#define MGBFS_SORTED_SCRATCH_BYTES (64ull << 20)
+int mgbfs_sorted_layer_noalloc(const mgbfs_sorted_layer_args* args, void* stream);
 /* Host-only: bank values (host memory) -> nibble-packed keys. */
 int mgbfs_sorted_unpack(uint32_t degree, const uint64_t* values, uint64_t count, uint64_t* keys);
 

@@ -14,6 +14,7 @@
 // point it allocates nothing and never synchronises; overflows are reported
 // through the sticky error word and the caller repeats the layer.
 #include <cstring>
+#include <mutex>
 #include "../../include/mgbfs_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
@@ -136,6 +137,47 @@ bool sorted_layer_check(const mgbfs_sorted_layer_args* a, LayerPlan& p) {
 
 }  // namespace
 
+// Scratch of mgbfs_sorted_layer_noalloc: one array per GPU, loaded with the
+// code object and held until the process ends.  Its name is a host-side
+// stand-in; kernels get the device address that hipGetSymbolAddress
+// returns for the device, looked up once per device.
+static_assert(MGBFS_SORTED_SCRATCH_BYTES % 256 == 0, "scratch size keeps 256-byte alignment");
+static __device__ __align__(256) uint8_t scratch[MGBFS_SORTED_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the stream's device (the current device
+// for the null stream), looked up with that device current.
+hipError_t scratch_address(hipStream_t stream, uint8_t** out) {
+  int current = 0, device = 0;
+  hipError_t e = hipGetDevice(&current);
+  if (e != hipSuccess) return e;
+  device = current;
+  if (stream && (e = hipStreamGetDevice(stream, &device)) != hipSuccess) return e;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    if (device != current && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (device != current) {
+      const hipError_t r = hipSetDevice(current);
+      if (e == hipSuccess) e = r;
+    }
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = static_cast<uint8_t*>(g_scratch_ptr[device]);
+  return hipSuccess;
+}
+
+}  // namespace
+
 }  // namespace mgbfs
 
 using namespace mgbfs;
@@ -171,6 +213,16 @@ extern "C" int mgbfs_sorted_layer(const mgbfs_sorted_layer_args* args, void* wor
       reinterpret_cast<uintptr_t>(workspace) % kAlign)
     return hipErrorInvalidValue;
   return sorted_layer_enqueue(*args, p, static_cast<uint8_t*>(workspace), static_cast<hipStream_t>(stream));
+}
+
+extern "C" int mgbfs_sorted_layer_noalloc(const mgbfs_sorted_layer_args* args, void* stream) {
+  LayerPlan p;
+  if (!sorted_layer_check(args, p)) return hipErrorInvalidValue;
+  if (p.bytes > MGBFS_SORTED_SCRATCH_BYTES) return hipErrorOutOfMemory;
+  uint8_t* ws = nullptr;
+  const hipError_t e = scratch_address(static_cast<hipStream_t>(stream), &ws);
+  if (e != hipSuccess) return e;
+  return sorted_layer_enqueue(*args, p, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int mgbfs_sorted_unpack(uint32_t degree, const uint64_t* values, uint64_t count, uint64_t* keys) {
