@@ -86,7 +86,7 @@ typedef struct mgbfs_engine_options {
   uint64_t capacity;     /* max states per layer (0 = plan from free VRAM)     */
   uint64_t vram_budget;  /* bytes the engine may allocate (0 = free - reserve) */
   uint32_t flags;        /* MGBFS_FLAG_* (0 for normal runs)                   */
-  uint32_t reserved;
+  uint32_t batch;        /* generic backend: parents per expansion launch (0 = 2^20) */
 } mgbfs_engine_options;
 
 typedef struct mgbfs_engine_info {
@@ -105,6 +105,28 @@ typedef struct mgbfs_engine_info {
   uint64_t algorithmic_bytes; /* modelled streaming HBM bytes of those kernels */
   uint64_t retries;        /* sorted backend: layer passes redone at finer buckets */
 } mgbfs_engine_info;
+
+/* Memory plan of a graph on one device, computed without allocating: the
+ * admission figures of the reference's GenericMemoryPlan /
+ * GenericDistributedMemoryPlan (crates/mgbfs-runtime/src/generic_memory.rs:
+ * 36-49, generic_distributed_memory.rs:7-13) for the backend this library
+ * would pick.  With options->vram_budget = 0 the budget is the device's free
+ * memory minus the 2 GiB (or 256 MiB, MGBFS_FLAG_MAX_CAPACITY) reserve. */
+typedef struct mgbfs_engine_plan_info {
+  uint32_t backend;          /* MGBFS_BACKEND_* chosen                               */
+  uint32_t history_layers;   /* retention rule: 3 inverse-closed, 1 full history     */
+  uint32_t retained_banks;   /* layer banks kept (sorted: 2 with back-edge masks)    */
+  uint32_t state_bytes;      /* bytes per stored element (8 packed/int64, 1 u8 SoA)  */
+  uint64_t capacity;         /* states per layer (packed) or arena rows (generic)    */
+  uint64_t batch;            /* parents expanded per launch                          */
+  uint64_t queue_capacity;   /* children generated per launch at most                */
+  uint64_t table_slots;      /* visited-set slots (hash slots; bitmap: bits; sorted: 0) */
+  uint64_t device_bytes;     /* planned device bytes                                 */
+  uint64_t budget_bytes;     /* the budget the plan was sized against                */
+  uint64_t state_bound;      /* state-space bound (0 = unknown)                      */
+} mgbfs_engine_plan_info;
+int mgbfs_engine_plan(const mgbfs_graph_desc* graph, const mgbfs_engine_options* options,
+                      mgbfs_engine_plan_info* plan, char* error_text, size_t error_capacity);
 
 /* ------------------------------------------------------------ engine */
 /* Host runtime of the single-GPU path; replaces GenericNativeBfs
@@ -129,6 +151,11 @@ int mgbfs_engine_phases(void* engine, double* seconds, uint64_t* bytes, uint32_t
 /* Clears the visited state and re-seeds the start state, keeping every
  * allocation (repeat runs of the same graph, e.g. timing loops). */
 int mgbfs_engine_reset(void* engine);
+/* Switches the engine to another graph of the same shape (packed backends:
+ * same degree; the sorted backend also needs an inverse-closed set) and
+ * resets it, keeping every device allocation.  For sweeps and resident
+ * workers that run many graphs of one degree. */
+int mgbfs_engine_rebind(void* engine, const mgbfs_graph_desc* graph);
 /* Layer sizes so far, depth 0 first (the report.json `layer_sizes`). */
 int mgbfs_engine_layer_sizes(void* engine, uint64_t* sizes, uint32_t capacity, uint32_t* count);
 /* Copies up to `limit` states of the current layer (which = 0) or of the
@@ -326,6 +353,10 @@ int mgbfs_generic_generate_u8(uint32_t kind, uint32_t elements, uint32_t rows, u
                               uint8_t* output, uint32_t output_stride, uint32_t* device_error, void* raw_stream);
 
 /* ---------------------------------------------------------- versions */
+/* Device properties in the reference's order (cuda/generic_graph.cu:78-82):
+ * values[8] = {compute units, compute major, compute minor, L2 bytes, wave
+ * size, max threads per CU, max threads per workgroup, LDS bytes per CU}. */
+int mgbfs_generic_hardware_info(int device, uint64_t* values);
 uint32_t mgbfs_hip_abi_version(void);
 /* Device properties: CU count, wave size, LDS bytes per workgroup, HBM bytes. */
 int mgbfs_hip_device_info(int32_t device, uint32_t* compute_units, uint32_t* wave_size, uint32_t* lds_bytes,

@@ -31,7 +31,7 @@ class GraphDesc(C.Structure):
 
 class EngineOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("backend", C.c_uint32), ("capacity", C.c_uint64),
-                ("vram_budget", C.c_uint64), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("vram_budget", C.c_uint64), ("flags", C.c_uint32), ("batch", C.c_uint32)]
 
 
 class EngineInfo(C.Structure):
@@ -70,6 +70,8 @@ def load():
     lib.mgbfs_engine_phases.restype = C.c_int
     lib.mgbfs_engine_reset.argtypes = [p]
     lib.mgbfs_engine_reset.restype = C.c_int
+    lib.mgbfs_engine_rebind.argtypes = [p, C.POINTER(GraphDesc)]
+    lib.mgbfs_engine_rebind.restype = C.c_int
     lib.mgbfs_engine_layer_sizes.argtypes = [p, C.POINTER(u64), u32, C.POINTER(u32)]
     lib.mgbfs_engine_layer_sizes.restype = C.c_int
     lib.mgbfs_engine_sample.argtypes = [p, u32, u64, C.POINTER(C.c_int64), C.POINTER(u64)]
@@ -99,32 +101,12 @@ class NativeEngine:
 
     def __init__(self, graph: dict, *, device: int = 0, backend: str | int = "auto", capacity: int = 0,
                  vram_budget: int = 0, inverse_closed: bool | None = None, flags: int = 0):
-        from .graph_definition import GraphDefinition
         lib = load()
-        gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
-        data = gd.to_dict()
-        a = data["action"]
-        self.elements = len(data["start"])
-        self._keep = []
-        desc = GraphDesc()
-        desc.elements = self.elements
-        desc.generators = len(a["generators"])
-        desc.inverse_closed = int(gd.inverse_closed() if inverse_closed is None else inverse_closed)
-        desc.expected_states = data["expected_max_unique_states"] or 0
-        start = _array(C.c_int64, data["start"])
-        desc.start = start
-        self._keep.append(start)
-        if a["kind"] == "permutation":
-            desc.kind, desc.rows, desc.cols = 0, self.elements, 1
-            perms = _array(C.c_uint32, [v for g in a["generators"] for v in g])
-            desc.permutations = perms
-            self._keep.append(perms)
-        else:
-            desc.kind, desc.rows, desc.cols = 1, a["rows"], a["cols"]
-            mats = _array(C.c_int64, [v for g in a["generators"] for v in g["matrix"]])
-            mods = _array(C.c_uint32, [g["modulo"] for g in a["generators"]])
-            desc.matrices, desc.moduli = mats, mods
-            self._keep += [mats, mods]
+        desc = graph_desc(graph)
+        self.elements = desc.elements
+        if inverse_closed is not None:
+            desc.inverse_closed = int(inverse_closed)
+        self._keep = [desc]
         opts = EngineOptions()
         opts.device = device
         opts.backend = BACKENDS[backend] if isinstance(backend, str) else int(backend)
@@ -169,6 +151,17 @@ class NativeEngine:
             raise RuntimeError(f"ENGINE_RUN_{rc}: {self._lib.mgbfs_engine_backend_name(None).decode()}")
         self.status = rc
         return rc
+
+    def rebind(self, graph):
+        """Switch to another graph of the same shape, keeping the engine's
+        device allocations (mgbfs_engine_rebind), and reset."""
+        desc = graph_desc(graph)
+        rc = self._lib.mgbfs_engine_rebind(self._h, C.byref(desc))
+        if rc:
+            raise RuntimeError(f"ENGINE_REBIND_{rc}: {self._lib.mgbfs_engine_backend_name(None).decode()}")
+        self._keep = [desc]
+        self.status = None
+        self.fatal = 0
 
     def reset(self):
         rc = self._lib.mgbfs_engine_reset(self._h)

@@ -101,17 +101,11 @@ static void unpack_rows(const std::vector<uint64_t>& keys, uint32_t n, std::vect
 
 class BitmapEngine final : public Engine {
  public:
-  BitmapEngine(const Graph& g, const mgbfs_engine_options& o) : gens_(g.packed_generators()), n_(g.elements) {
+  BitmapEngine(const Graph& g, const mgbfs_engine_options& o) : n_(g.elements) {
     device_ = o.device;
-    // Parent-edge tags live in nibble 15 when the state uses at most 15.
-    if (n_ <= 15 && gens_.count <= 14) {
-      const std::vector<int> inv = inverse_index(gens_);
-      gens_.tag_shift = 60;
-      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_tag[a] = inv[a] >= 0 ? static_cast<uint8_t>(inv[a] + 1) : 0;
-    }
+    set_generators(g);
     key_mask_ = n_ >= 16 ? ~0ull : ((1ull << (4 * n_)) - 1);
     backend = MGBFS_BACKEND_BITMAP;
-    history_layers = g.inverse_closed ? 3 : 0;
     MGBFS_HIP(hipSetDevice(device_));
     MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     MGBFS_HIP(hipEventCreate(&ev0_));
@@ -132,7 +126,6 @@ class BitmapEngine final : public Engine {
     if (cap_limit_ == 0) throw std::runtime_error("BITMAP_VRAM_CAPACITY");
     grow(front_, front_cap_, std::min<uint64_t>(cap_limit_, 1u << 16));
     grow(next_, next_cap_, std::min<uint64_t>(cap_limit_, 1u << 16));
-    start_key_ = g.start_key();
     reset();
   }
 
@@ -230,6 +223,13 @@ class BitmapEngine final : public Engine {
     return MGBFS_LAYER;
   }
 
+  // Another graph of the same degree reuses the bitmap and frontiers.
+  void rebind(const Graph& g) override {
+    if (g.elements != n_ || !g.packable() || !g.distinct_perm()) throw std::runtime_error("ENGINE_REBIND_SHAPE");
+    set_generators(g);
+    reset();
+  }
+
   void layer_keys(std::vector<uint64_t>& out) override { copy_keys(front_, count_, count_, out); }
 
   void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) override {
@@ -251,6 +251,18 @@ class BitmapEngine final : public Engine {
 
  private:
   uint32_t* error_word() { return reinterpret_cast<uint32_t*>(control_.as<uint64_t>() + 1); }
+  // Generator table, parent-edge tags (nibble 15, when the state uses at
+  // most 15 nibbles) and start state of a graph.
+  void set_generators(const Graph& g) {
+    gens_ = g.packed_generators();
+    if (n_ <= 15 && gens_.count <= 14) {
+      const std::vector<int> inv = inverse_index(gens_);
+      gens_.tag_shift = 60;
+      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_tag[a] = inv[a] >= 0 ? static_cast<uint8_t>(inv[a] + 1) : 0;
+    }
+    history_layers = g.inverse_closed ? 3 : 0;
+    start_key_ = g.start_key();
+  }
   uint64_t total() const {
     uint64_t t = 0;
     for (uint64_t s : layers) t += s;
@@ -402,8 +414,8 @@ uint64_t state_space_bound(const Graph& g) {
   return bound;
 }
 
-static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
-  uint32_t backend = o.backend;
+static uint32_t choose_backend(const Graph& g, uint32_t requested) {
+  uint32_t backend = requested;
   if (backend == MGBFS_BACKEND_AUTO) {
     // The sorted backend keeps at least 4 tag bits beside the 4n key bits.
     if (g.packable() && g.inverse_closed && g.elements <= 15) backend = MGBFS_BACKEND_SORTED;
@@ -413,14 +425,90 @@ static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
   switch (backend) {
     case MGBFS_BACKEND_BITMAP:
       if (!g.packable() || !g.distinct_perm()) throw std::runtime_error("BITMAP_UNSUPPORTED_GRAPH");
-      return make_bitmap_engine(g, o);
+      return backend;
     case MGBFS_BACKEND_SORTED:
       if (!g.packable() || g.elements > 15) throw std::runtime_error("SORTED_UNSUPPORTED_GRAPH");
-      return make_sorted_engine(g, o);
+      if (!g.inverse_closed) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
+      return backend;
     case MGBFS_BACKEND_GENERIC:
-      return make_generic_engine(g, o, state_space_bound(g));
+      return backend;
     default:
       throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
+  }
+}
+
+// Planned bytes per state of the widest layer on the sorted backend: three
+// banks grown by a quarter, plus per child (G-1 with back-edge masks, else
+// G) a sub-bucket slab entry (8 B at the mean fill of 5000 of 5120) and an
+// owner-lane slab entry with its slack (10 B).
+static uint64_t sorted_bytes_per_state(const Graph& g) {
+  const uint32_t G = g.generators;
+  const bool masks = G + 2 <= 64 - 4 * g.elements;
+  const uint32_t children = masks && G > 1 ? G - 1 : G;
+  return 30 + 18ull * children;
+}
+
+mgbfs_engine_plan_info plan_engine(const Graph& g, const mgbfs_engine_options& o) {
+  mgbfs_engine_plan_info p{};
+  p.backend = choose_backend(g, o.backend);
+  p.history_layers = g.inverse_closed ? 3 : 1;
+  p.state_bound = state_space_bound(g);
+  if (!o.vram_budget) MGBFS_HIP(hipSetDevice(o.device));  // a given budget needs no device
+  p.budget_bytes = vram_budget(o);
+  const uint64_t G = g.generators;
+  switch (p.backend) {
+    case MGBFS_BACKEND_BITMAP: {
+      // BitmapEngine: n!-bit bitmap plus two frontier arrays of capacity rows.
+      const uint64_t domain = factorial_u64(g.elements);
+      const uint64_t bitmap = (domain + 31) / 32 * 4;
+      if (bitmap + 64 * 1024 > p.budget_bytes) throw std::runtime_error("BITMAP_VRAM_CAPACITY");
+      p.state_bytes = 8;
+      p.retained_banks = 2;
+      p.capacity = std::min<uint64_t>(o.capacity ? o.capacity : domain, (p.budget_bytes - bitmap) / 16);
+      p.batch = p.capacity;
+      p.table_slots = domain;
+      p.device_bytes = bitmap + 16 * p.capacity + 8000 + 64;
+      break;
+    }
+    case MGBFS_BACKEND_SORTED: {
+      const uint64_t per = sorted_bytes_per_state(g);
+      const uint64_t fixed = 5 * ((1ull << 20) + 1) * 4 + (1ull << 20);
+      if (fixed + per > p.budget_bytes) throw std::runtime_error("SORTED_VRAM_CAPACITY");
+      uint64_t cap = (p.budget_bytes - fixed) / per;
+      if (p.state_bound) cap = std::min(cap, p.state_bound);
+      cap = std::min<uint64_t>(cap, 0xfffffff0ull / std::max<uint64_t>(1, G - (G > 1 ? 1 : 0)));
+      if (o.capacity) cap = std::min(cap, o.capacity);
+      p.state_bytes = 8;
+      p.retained_banks = G + 2 <= 64 - 4 * g.elements ? 2 : 3;
+      p.capacity = cap;
+      p.batch = cap;
+      p.table_slots = 0;
+      p.device_bytes = fixed + per * cap;
+      break;
+    }
+    default: {
+      p.state_bytes = generic_state_bytes(g);
+      p.retained_banks = 0;
+      p.capacity = o.capacity ? o.capacity : generic_capacity(g, p.budget_bytes, p.state_bound);
+      if (!p.capacity) throw std::runtime_error("GENERIC_VRAM_CAPACITY");
+      p.batch = generic_batch(g.generators, o.batch ? std::min<uint64_t>(o.batch, p.capacity) : p.capacity);
+      p.device_bytes = generic_plan_bytes(g.elements, p.capacity, p.state_bytes, &p.table_slots) +
+                       g.perms.size() * 4 + g.matrices.size() * 8 + g.moduli.size() * 4;
+      break;
+    }
+  }
+  p.queue_capacity = p.batch * G;
+  return p;
+}
+
+static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
+  switch (choose_backend(g, o.backend)) {
+    case MGBFS_BACKEND_BITMAP:
+      return make_bitmap_engine(g, o);
+    case MGBFS_BACKEND_SORTED:
+      return make_sorted_engine(g, o);
+    default:
+      return make_generic_engine(g, o, state_space_bound(g));
   }
 }
 
@@ -462,6 +550,24 @@ extern "C" int mgbfs_engine_create(const mgbfs_graph_desc* graph, const mgbfs_en
     *engine = mgbfs::create(g, *options);
     return 0;
   } catch (const std::exception& e) {
+    put_error(error_text, error_capacity, e.what());
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+extern "C" int mgbfs_engine_plan(const mgbfs_graph_desc* graph, const mgbfs_engine_options* options,
+                                 mgbfs_engine_plan_info* plan, char* error_text, size_t error_capacity) {
+  if (!plan || !options) return hipErrorInvalidValue;
+  int old = 0;
+  (void)hipGetDevice(&old);
+  try {
+    const mgbfs::Graph g = mgbfs::intake(graph);
+    *plan = mgbfs::plan_engine(g, *options);
+    (void)hipSetDevice(old);
+    return 0;
+  } catch (const std::exception& e) {
+    (void)hipSetDevice(old);
     put_error(error_text, error_capacity, e.what());
     g_last_error = e.what();
     return -1;
@@ -526,6 +632,11 @@ extern "C" int mgbfs_engine_reset(void* engine) {
     static_cast<Engine*>(engine)->reset();
     return 0;
   })
+}
+
+extern "C" int mgbfs_engine_rebind(void* engine, const mgbfs_graph_desc* graph) {
+  if (!engine) return hipErrorInvalidValue;
+  MGBFS_GUARD(static_cast<Engine*>(engine)->rebind(mgbfs::intake(graph)); return 0;)
 }
 
 extern "C" int mgbfs_engine_layer_sizes(void* engine, uint64_t* sizes, uint32_t capacity, uint32_t* count) {

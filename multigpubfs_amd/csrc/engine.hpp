@@ -75,6 +75,9 @@ class Engine {
   virtual ~Engine() = default;
   virtual int advance(uint64_t* next_count, uint32_t* fatal) = 0;
   virtual void reset() = 0;
+  // Switches to another graph of the same shape, keeping every allocation,
+  // and resets (backends that support it).
+  virtual void rebind(const Graph&) { throw std::runtime_error("ENGINE_REBIND_UNSUPPORTED"); }
   virtual void layer_keys(std::vector<uint64_t>& out) = 0;
   virtual void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) = 0;
   virtual const char* name() const = 0;
@@ -114,5 +117,12 @@ uint64_t vram_budget(const mgbfs_engine_options& o);
 uint64_t state_space_bound(const Graph& g);
 // Index of each generator's inverse in the set (-1 when absent).
 std::vector<int> inverse_index(const PackedGenerators& gens);
+// Generic backend sizing (generic_engine.hip).
+uint64_t generic_plan_bytes(uint32_t elements, uint64_t capacity, uint32_t state_bytes, uint64_t* slots_out);
+uint32_t generic_state_bytes(const Graph& g);
+uint64_t generic_capacity(const Graph& g, uint64_t budget, uint64_t state_bound);
+uint32_t generic_batch(uint32_t generators, uint64_t capacity);
+// Backend choice and memory plan of a graph, without allocating.
+mgbfs_engine_plan_info plan_engine(const Graph& g, const mgbfs_engine_options& o);
 
 }  // namespace mgbfs

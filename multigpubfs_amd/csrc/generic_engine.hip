@@ -42,11 +42,36 @@ constexpr uint32_t kBatch = 1u << 20;
 }  // namespace
 
 // Device bytes of a plan (generic_memory.rs:36-41).
-static uint64_t generic_plan_bytes(uint32_t elements, uint64_t capacity, uint32_t state_bytes, uint64_t* slots_out) {
+uint64_t generic_plan_bytes(uint32_t elements, uint64_t capacity, uint32_t state_bytes, uint64_t* slots_out) {
   uint64_t slots = 1;
   while (slots < 2 * capacity) slots <<= 1;
   if (slots_out) *slots_out = slots;
   return static_cast<uint64_t>(elements) * capacity * state_bytes * 2 + capacity * 8 + slots * 8 + 24;
+}
+
+uint32_t generic_state_bytes(const Graph& g) {
+  return g.kind == 0 && std::all_of(g.start.begin(), g.start.end(), [](int64_t v) { return v >= 0 && v <= 255; })
+             ? 1u
+             : 8u;
+}
+
+// Largest arena capacity whose plan fits `budget` (generic_native.rs:59-64),
+// capped by the state-space bound; 0 when not even one row fits.
+uint64_t generic_capacity(const Graph& g, uint64_t budget, uint64_t state_bound) {
+  const uint32_t sb = generic_state_bytes(g);
+  uint64_t lo = 1, hi = std::min<uint64_t>(kMaxArena, state_bound ? state_bound : kMaxArena);
+  if (generic_plan_bytes(g.elements, 1, sb, nullptr) > budget) return 0;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo + 1) / 2;
+    if (generic_plan_bytes(g.elements, mid, sb, nullptr) <= budget) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+uint32_t generic_batch(uint32_t generators, uint64_t capacity) {
+  const uint64_t b = std::max<uint32_t>(1, std::min<uint32_t>(kBatch, 0x7ffffffeu / generators));
+  return static_cast<uint32_t>(std::min<uint64_t>(b, capacity));
 }
 
 class GenericEngine final : public Engine {
@@ -59,22 +84,12 @@ class GenericEngine final : public Engine {
     MGBFS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     MGBFS_HIP(hipEventCreate(&ev0_));
     MGBFS_HIP(hipEventCreate(&ev1_));
-    state_bytes = 8;
-    if (g.kind == 0 && std::all_of(g.start.begin(), g.start.end(), [](int64_t v) { return v >= 0 && v <= 255; }))
-      state_bytes = 1;
+    state_bytes = generic_state_bytes(g);
     elements_ = g.elements;
     uint64_t cap = o.capacity;
     if (!cap) {
-      // Largest capacity whose plan fits the budget (generic_native.rs:59-64).
-      const uint64_t budget = vram_budget(o);
-      uint64_t lo = 1, hi = std::min<uint64_t>(kMaxArena, state_bound ? state_bound : kMaxArena);
-      if (generic_plan_bytes(elements_, 1, state_bytes, nullptr) > budget) throw std::runtime_error("GENERIC_VRAM_CAPACITY");
-      while (lo < hi) {
-        const uint64_t mid = lo + (hi - lo + 1) / 2;
-        if (generic_plan_bytes(elements_, mid, state_bytes, nullptr) <= budget) lo = mid;
-        else hi = mid - 1;
-      }
-      cap = lo;
+      cap = generic_capacity(g, vram_budget(o), state_bound);
+      if (!cap) throw std::runtime_error("GENERIC_VRAM_CAPACITY");
     }
     if (cap == 0 || cap > kMaxArena) throw std::runtime_error("GENERIC_ARENA_SHAPE");
     cap_ = static_cast<uint32_t>(cap);
@@ -96,6 +111,7 @@ class GenericEngine final : public Engine {
       MGBFS_HIP(hipMemcpy(mods_.as<void>(), g.moduli.data(), g.moduli.size() * 4, hipMemcpyHostToDevice));
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), 64, hipHostMallocDefault));
     hash_bits_ = (o.flags & MGBFS_FLAG_TEST_COLLIDE_HASH) ? 0u : 64u;
+    batch_ = generic_batch(g.generators, o.batch ? o.batch : kMaxArena);
     device_bytes = visited_.bytes() + parents_.bytes() + front_.bytes() + future_.bytes() + table_.bytes() +
                    control_.bytes() + perms_.bytes() + mats_.bytes() + mods_.bytes();
     reset();
@@ -154,7 +170,7 @@ class GenericEngine final : public Engine {
     MGBFS_HIP(hipEventRecord(ev0_, stream_));
     gather(front_.as<uint32_t>(), count_, parents_, cap_);
     MGBFS_HIP(fill_u32(control_.as<uint32_t>() + 1, 0u, 1, stream_));
-    const uint32_t batch = std::max<uint32_t>(1, std::min<uint32_t>(kBatch, 0x7ffffffeu / G));
+    const uint32_t batch = batch_;
     for (uint32_t begin = 0; begin < count_; begin += batch) {
       const uint32_t c = std::min(batch, count_ - begin);
       ++launches;
@@ -260,7 +276,7 @@ class GenericEngine final : public Engine {
   uint32_t elements_ = 0, cap_ = 0, hash_bits_ = 64;
   uint64_t slots_ = 0;
   DeviceBuffer visited_, parents_, front_, future_, table_, control_, perms_, mats_, mods_;
-  uint32_t count_ = 0, visited_used_ = 0, current_start_ = 0, prev_start_ = 0, prev_count_ = 0;
+  uint32_t count_ = 0, visited_used_ = 0, current_start_ = 0, prev_start_ = 0, prev_count_ = 0, batch_ = 1;
   uint32_t* host_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };

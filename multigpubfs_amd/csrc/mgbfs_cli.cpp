@@ -4,7 +4,10 @@
 // (crates/mgbfs-cli/src/main.rs:20-199, crates/mgbfs-runtime/src/generic_run.rs:9-43,
 // generic_distributed_run.rs:13-73):
 //   mgbfs graph <graph.json> <out-dir> [--device N] [--capacity N] [--seconds N] [--backend NAME]
-//   mgbfs graph-info <graph.json> <devices> [shards] [capacity] [batch]
+//   mgbfs graph-info <graph.json> <devices> <shards> [capacity|auto] [batch]
+//   mgbfs graph-plan <graph.json> <inventory.json> <shards> [capacity|auto] [batch]
+//   mgbfs graph-local-plan <graph.json> <device> <world> <shards> <capacity|auto> <batch>
+//   mgbfs graph-rank <graph.json> <launch.json> <rank> <bootstrap> <out-dir> <seconds>   (world 1)
 //   mgbfs graph-count
 //   mgbfs graph-digest <graph.json>      (validation, digest, retention rule; no device)
 //   mgbfs key-info
@@ -47,6 +50,7 @@ struct CliError : std::runtime_error {
 };
 
 const int64_t kI64Max = INT64_MAX;
+const uint64_t kReserveBytes = 2ull << 30;  // cold admission reserve (generic_memory.rs:9-19)
 
 int64_t wrap_mul_add(int64_t acc, int64_t a, int64_t b) {
   return static_cast<int64_t>(static_cast<uint64_t>(acc) + static_cast<uint64_t>(a) * static_cast<uint64_t>(b));
@@ -275,6 +279,7 @@ struct EngineCache {
   std::string key;
   void* engine = nullptr;
   uint64_t device_bytes = 0;
+  mgbfs_engine_plan_info plan{};
   void release() {
     if (engine) mgbfs_engine_destroy(engine);
     engine = nullptr;
@@ -289,52 +294,121 @@ bool path_exists(const std::string& path) {
   return !path.empty() && stat(path.c_str(), &st) == 0;
 }
 
-// One `graph` job: writes <out>/states.json and <out>/report.json and returns
-// the report.  default_device applies when no --device is given; with a
-// cache the engine survives the job; cancel_path (if it appears between
-// layers) ends the run INCOMPLETE / CANCELLED like SIGTERM.
-Json run_graph_job(const std::vector<std::string>& args, int32_t default_device, EngineCache* cache,
-                   const std::string& cancel_path, bool* reused) {
-  if (args.size() < 2)
-    throw CliError(
-        "CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N] "
-        "[--archive PATH]");
-  int32_t device = default_device;
-  uint64_t capacity = 0, seconds = 0;
-  std::string backend = "auto";
-  std::string archive_path;
-  uint32_t flags = 0;
-  for (size_t i = 2; i < args.size(); i += 2) {
-    if (i + 1 >= args.size()) throw CliError("CLI_GRAPH_OPTION_VALUE");
-    const std::string& k = args[i];
-    if (k == "--device") device = static_cast<int32_t>(parse_u64(args[i + 1], "CLI_GRAPH_DEVICE"));
-    else if (k == "--capacity") capacity = parse_u64(args[i + 1], "CLI_GRAPH_CAPACITY");
-    else if (k == "--seconds") seconds = parse_u64(args[i + 1], "CLI_GRAPH_SECONDS");
-    else if (k == "--backend") backend = args[i + 1];
-    else if (k == "--archive") archive_path = args[i + 1];
-    else if (k == "--memory-preset") {
-      if (args[i + 1] == "max_capacity") flags |= MGBFS_FLAG_MAX_CAPACITY;
-      else if (args[i + 1] != "balanced") throw CliError("INVALID_MEMORY_PRESET");
-    } else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
-  }
-  if (cache && device != default_device) throw CliError("GENERIC_SESSION_DEVICE");
-  const char* world = std::getenv("WORLD_SIZE");
-  if (world && std::string(world) != "1") throw CliError("GENERIC_DISTRIBUTED_INTEGRATION_NOT_READY");
-  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
-  const std::string out = args[1];
-  if (path_exists(out)) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
-  if (mkdir(out.c_str(), 0755)) throw CliError("CLI_GRAPH_OUTPUT_CREATE");
+// Reference names of this library's visited-set algorithms and layouts.
+const char* history_algorithm(uint32_t backend) {
+  return backend == MGBFS_BACKEND_BITMAP ? "PERMUTATION_BITMAP" : backend == MGBFS_BACKEND_SORTED ? "SORTED_RUNS" : "HASH";
+}
+const char* table_layout(uint32_t backend) {
+  return backend == MGBFS_BACKEND_BITMAP   ? "LEHMER_RANK_BITMAP"
+         : backend == MGBFS_BACKEND_SORTED ? "SUB_BUCKET_LDS_TABLES"
+                                           : "OPEN_ADDRESSING_U64_SLOTS";
+}
+const char* backend_key(uint32_t backend) {
+  return backend == MGBFS_BACKEND_BITMAP ? "permutation_bitmap" : backend == MGBFS_BACKEND_SORTED ? "sorted" : "generic";
+}
 
+// Generator table bytes as the reference counts them (generic_distributed_run.rs:10).
+uint64_t generator_bytes(const GraphDoc& g) {
+  uint64_t b = 0;
+  for (const auto& p : g.perms) b += 4 * p.size();
+  for (const auto& m : g.mats) b += 8 * m.size() + 4;
+  return b;
+}
+
+// A rank plan with the fields of GenericDistributedMemoryPlan
+// (crates/mgbfs-runtime/src/generic_distributed_memory.rs:7-13), from this
+// library's engine plan.
+Json plan_json(const GraphDoc& graph, const mgbfs_engine_plan_info& p, uint32_t world, uint32_t shards) {
+  Json plan = Json::object();
+  plan["generator_backend"] = Json::str("hip");
+  plan["history_algorithm"] = Json::str(history_algorithm(p.backend));
+  plan["owner_lanes"] = Json::integer(0);
+  plan["sorted_owner_bytes"] = Json::integer(0);
+  plan["driver_graph_reserve_bytes"] = Json::integer(0);
+  plan["parent_transport"] = Json::boolean(false);
+  plan["sort_candidates"] = Json::boolean(p.backend == MGBFS_BACKEND_SORTED);
+  plan["history_layers"] = Json::integer(p.history_layers);
+  plan["retained_banks"] = Json::integer(p.retained_banks);
+  plan["state_bytes"] = Json::integer(p.state_bytes);
+  plan["elements"] = Json::integer(graph.elements);
+  plan["world"] = Json::integer(world);
+  plan["shards"] = Json::integer(shards);
+  plan["capacity"] = Json::unsigned_(p.capacity);
+  plan["batch"] = Json::unsigned_(p.batch);
+  plan["table_layout"] = Json::str(table_layout(p.backend));
+  plan["queue_capacity"] = Json::unsigned_(p.queue_capacity);
+  plan["table_slots"] = Json::unsigned_(p.table_slots);
+  plan["generator_bytes"] = Json::unsigned_(generator_bytes(graph));
+  plan["device_bytes"] = Json::unsigned_(p.device_bytes);
+  plan["state_bound"] = Json::unsigned_(p.state_bound);
+  plan["backend"] = Json::str(backend_key(p.backend));
+  return plan;
+}
+
+void write_job_phase(const std::string& path, const char* phase) {
+  if (path.empty()) return;
+  const double unix_s = std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch()).count();
+  Json v = Json::object();
+  if (path_exists(path)) {
+    try {
+      v = mgbfs::host::JsonParser(read_file(path)).parse();
+    } catch (const std::exception&) {
+      v = Json::object();
+    }
+  }
+  if (v.kind != Json::Obj || !v.has("entered")) {
+    v = Json::object();
+    v["schema"] = Json::integer(1);
+    v["pid"] = Json::integer(static_cast<int64_t>(getpid()));
+    const char* seq = std::getenv("MGBFS_JOB_SEQUENCE");
+    v["sequence"] = Json::str(seq ? seq : "");
+    v["entered"] = Json::object();
+  }
+  v["phase"] = Json::str(phase);
+  v["entered"][phase] = Json::real(unix_s);
+  v["updated_unix"] = Json::real(unix_s);
+  write_file(path, mgbfs::host::dumps(v));
+}
+
+// What one BFS run asks for.
+struct BfsRequest {
+  int32_t device = 0;
+  uint64_t capacity = 0, seconds = 0;
+  uint32_t batch = 0, flags = 0;
+  std::string backend = "auto", archive_path, phase_path;
+};
+
+// What it produced: layer table, stop reason, timings, engine figures and
+// the states.json text.
+struct BfsOutcome {
+  std::vector<uint64_t> sizes;
+  std::vector<double> times;
+  std::string status = "COMPLETE", reason = "FRONTIER_EXHAUSTED", backend;
+  double setup_seconds = 0, bfs_seconds = 0;
+  mgbfs_engine_info info{};
+  mgbfs_engine_plan_info plan{};
+  std::string states_text;
+};
+
+// Runs one BFS to completion, a stop (deadline, cancel, capacity) or a
+// failure.  With a cache the engine survives the run; cancel_path (if it
+// appears between layers) ends the run INCOMPLETE / CANCELLED like SIGTERM.
+// sample_key names the states.json sample-limit field.
+BfsOutcome run_bfs(const GraphDoc& graph, const BfsRequest& req, EngineCache* cache, const std::string& cancel_path,
+                   bool* reused, const char* sample_key) {
+  BfsOutcome r;
   std::vector<uint32_t> pflat;
   std::vector<int64_t> mflat;
   const mgbfs_graph_desc desc = graph.desc(pflat, mflat);
   mgbfs_engine_options opts{};
-  opts.device = device;
-  opts.backend = backend_code(backend);
-  opts.capacity = capacity;
-  opts.flags = flags;
-  const std::string key = graph.digest() + "|" + std::to_string(device) + "|" + backend + "|" +
-                          std::to_string(capacity) + "|" + std::to_string(flags);
+  opts.device = req.device;
+  opts.backend = backend_code(req.backend);
+  opts.capacity = req.capacity;
+  opts.flags = req.flags;
+  opts.batch = req.batch;
+  const std::string key = graph.digest() + "|" + std::to_string(req.device) + "|" + req.backend + "|" +
+                          std::to_string(req.capacity) + "|" + std::to_string(req.flags) + "|" +
+                          std::to_string(req.batch);
   void* engine = nullptr;
   char err[256] = {0};
   const auto setup0 = std::chrono::steady_clock::now();
@@ -348,16 +422,20 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     if (reused) *reused = true;
   } else {
     if (cache) cache->release();
+    if (mgbfs_engine_plan(&desc, &opts, &r.plan, err, sizeof err)) throw CliError(err[0] ? err : "ENGINE_PLAN");
     if (mgbfs_engine_create(&desc, &opts, &engine, err, sizeof err)) throw CliError(err[0] ? err : "ENGINE_CREATE");
     if (cache) {
       cache->engine = engine;
       cache->key = key;
+      cache->plan = r.plan;
     }
   }
-  const double setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - setup0).count();
+  if (cache) r.plan = cache->plan;
+  r.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - setup0).count();
   std::signal(SIGTERM, on_signal);
   std::signal(SIGINT, on_signal);
-  std::cerr << "MGBFS_GRAPH_READY device=" << device << std::endl;
+  std::cerr << "MGBFS_GRAPH_READY device=" << req.device << std::endl;
+  write_job_phase(req.phase_path, "bfs");
   std::unique_ptr<LayerArchive> archive;
   std::vector<int64_t> arows(1000 * static_cast<size_t>(graph.elements));
   auto archive_layer = [&](uint64_t depth) {
@@ -365,23 +443,21 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
     if (mgbfs_engine_sample(engine, 0, 1000, arows.data(), &n)) throw CliError("ARCHIVE_SAMPLE");
     archive->layer(depth, arows, n);
   };
-  if (!archive_path.empty()) {
+  if (!req.archive_path.empty()) {
     if (graph.kind != 0) throw CliError("ARCHIVE_STATE_WIDTH");
-    archive = std::make_unique<LayerArchive>(archive_path, graph.elements, graph.digest());
+    archive = std::make_unique<LayerArchive>(req.archive_path, graph.elements, graph.digest());
     archive_layer(0);
   }
-  std::vector<double> times;
-  std::string status = "COMPLETE", reason = "FRONTIER_EXHAUSTED";
   const auto t0 = std::chrono::steady_clock::now();
   while (true) {
     if (g_cancel || path_exists(cancel_path)) {
-      status = "INCOMPLETE";
-      reason = "CANCELLED";
+      r.status = "INCOMPLETE";
+      r.reason = "CANCELLED";
       break;
     }
-    if (seconds && std::chrono::steady_clock::now() - t0 >= std::chrono::seconds(seconds)) {
-      status = "INCOMPLETE";
-      reason = "DEADLINE";
+    if (req.seconds && std::chrono::steady_clock::now() - t0 >= std::chrono::seconds(req.seconds)) {
+      r.status = "INCOMPLETE";
+      r.reason = "DEADLINE";
       break;
     }
     const auto l0 = std::chrono::steady_clock::now();
@@ -394,29 +470,30 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
       throw CliError(std::string("ENGINE_ADVANCE: ") + mgbfs_engine_backend_name(nullptr));
     }
     if (s == MGBFS_LAYER) {
-      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count());
-      if (archive) archive_layer(times.size());
+      r.times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count());
+      if (archive) archive_layer(r.times.size());
       continue;
     }
     if (s == MGBFS_RESOURCE) {
-      status = "INCOMPLETE";
-      reason = "RESOURCE_" + std::to_string(fatal);
+      r.status = "INCOMPLETE";
+      r.reason = "RESOURCE_" + std::to_string(fatal);
     }
     break;
   }
-  const double bfs_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  r.bfs_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  write_job_phase(req.phase_path, "retiring");
   if (archive) archive->commit();
   uint32_t nl = 0;
   mgbfs_engine_layer_sizes(engine, nullptr, 0, &nl);
-  std::vector<uint64_t> sizes(nl);
-  mgbfs_engine_layer_sizes(engine, sizes.data(), nl, &nl);
-  mgbfs_engine_info info{};
-  mgbfs_engine_query(engine, &info);
+  r.sizes.resize(nl);
+  mgbfs_engine_layer_sizes(engine, r.sizes.data(), nl, &nl);
+  mgbfs_engine_query(engine, &r.info);
+  r.backend = mgbfs_engine_backend_name(engine);
   // Samples: <= 1000 rows of the last completed layer; the previous layer only if < 1000.
   std::vector<int64_t> cur(1000 * static_cast<size_t>(graph.elements)), prev;
   uint64_t ncur = 0, nprev = 0;
   if (mgbfs_engine_sample(engine, 0, 1000, cur.data(), &ncur)) ncur = 0;
-  if (nl >= 2 && sizes[nl - 2] < 1000) {
+  if (nl >= 2 && r.sizes[nl - 2] < 1000) {
     prev.resize(1000 * static_cast<size_t>(graph.elements));
     if (mgbfs_engine_sample(engine, 1, 1000, prev.data(), &nprev)) nprev = 0;
   }
@@ -425,46 +502,80 @@ Json run_graph_job(const std::vector<std::string>& args, int32_t default_device,
   states["state_encoding"] = Json::str("signed_int64_vectors");
   states["current"] = rows_json(cur, ncur, graph.elements);
   states["previous_small"] = rows_json(prev, nprev, graph.elements);
-  states["current_sample_limit"] = Json::integer(1000);
-  const std::string states_text = mgbfs::host::dumps(states);
+  states[sample_key] = Json::integer(1000);
+  r.states_text = mgbfs::host::dumps(states);
+  if (cache) cache->device_bytes = r.info.device_bytes;
+  else mgbfs_engine_destroy(engine);
+  return r;
+}
+
+Json layer_json(const BfsOutcome& r, Json& seconds) {
+  Json ls = Json::array();
+  for (uint64_t v : r.sizes) ls.push(Json::unsigned_(v));
+  seconds = Json::array();
+  for (double v : r.times) seconds.push(Json::real(v));
+  return ls;
+}
+
+// One `graph` job: writes <out>/states.json and <out>/report.json and returns
+// the report.  default_device applies when no --device is given.
+Json run_graph_job(const std::vector<std::string>& args, int32_t default_device, EngineCache* cache,
+                   const std::string& cancel_path, bool* reused) {
+  if (args.size() < 2)
+    throw CliError(
+        "CLI_USAGE: mgbfs graph <graph.json> <output-dir> [--device N] [--capacity N] [--seconds N] "
+        "[--archive PATH]");
+  BfsRequest req;
+  req.device = default_device;
+  for (size_t i = 2; i < args.size(); i += 2) {
+    if (i + 1 >= args.size()) throw CliError("CLI_GRAPH_OPTION_VALUE");
+    const std::string& k = args[i];
+    if (k == "--device") req.device = static_cast<int32_t>(parse_u64(args[i + 1], "CLI_GRAPH_DEVICE"));
+    else if (k == "--capacity") req.capacity = parse_u64(args[i + 1], "CLI_GRAPH_CAPACITY");
+    else if (k == "--seconds") req.seconds = parse_u64(args[i + 1], "CLI_GRAPH_SECONDS");
+    else if (k == "--backend") req.backend = args[i + 1];
+    else if (k == "--archive") req.archive_path = args[i + 1];
+    else if (k == "--memory-preset") {
+      if (args[i + 1] == "max_capacity") req.flags |= MGBFS_FLAG_MAX_CAPACITY;
+      else if (args[i + 1] != "balanced") throw CliError("INVALID_MEMORY_PRESET");
+    } else throw CliError("CLI_GRAPH_UNKNOWN_OPTION");
+  }
+  if (cache && req.device != default_device) throw CliError("GENERIC_SESSION_DEVICE");
+  const char* world = std::getenv("WORLD_SIZE");
+  if (world && std::string(world) != "1") throw CliError("GENERIC_DISTRIBUTED_INTEGRATION_NOT_READY");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  const std::string out = args[1];
+  if (path_exists(out)) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
+  if (mkdir(out.c_str(), 0755)) throw CliError("CLI_GRAPH_OUTPUT_CREATE");
+  const BfsOutcome r = run_bfs(graph, req, cache, cancel_path, reused, "current_sample_limit");
 
   Json report = Json::object();
   report["schema"] = Json::integer(2);
-  report["status"] = Json::str(status);
-  report["reason"] = Json::str(reason);
-  report["backend"] = Json::str(mgbfs_engine_backend_name(engine));
-  report["device"] = Json::integer(device);
+  report["status"] = Json::str(r.status);
+  report["reason"] = Json::str(r.reason);
+  report["backend"] = Json::str(r.backend);
+  report["device"] = Json::integer(req.device);
   report["world"] = Json::integer(1);
   report["graph_digest"] = Json::str(graph.digest());
-  Json ls = Json::array();
-  for (uint64_t v : sizes) ls.push(Json::unsigned_(v));
-  report["layer_sizes"] = ls;
-  Json lt = Json::array();
-  for (double v : times) lt.push(Json::real(v));
-  report["layer_seconds"] = lt;
-  report["setup_seconds"] = Json::real(setup_seconds);
-  report["bfs_seconds"] = Json::real(bfs_seconds);
-  report["capacity"] = Json::unsigned_(capacity);
-  report["arena_bytes"] = Json::unsigned_(info.device_bytes);
-  Json plan = Json::object();
-  plan["state_bytes"] = Json::integer(info.state_bytes);
-  plan["elements"] = Json::integer(graph.elements);
-  plan["history_layers"] = Json::integer(info.history_layers);
-  plan["world"] = Json::integer(1);
-  plan["device_bytes"] = Json::unsigned_(info.device_bytes);
-  report["plan"] = plan;
+  Json seconds;
+  report["layer_sizes"] = layer_json(r, seconds);
+  report["layer_seconds"] = seconds;
+  report["setup_seconds"] = Json::real(r.setup_seconds);
+  report["bfs_seconds"] = Json::real(r.bfs_seconds);
+  report["capacity"] = Json::unsigned_(r.plan.capacity);
+  report["requested_capacity"] = Json::unsigned_(req.capacity);
+  report["arena_bytes"] = Json::unsigned_(r.info.device_bytes);
+  report["plan"] = plan_json(graph, r.plan, 1, 1);
   Json devices = Json::array();
-  devices.push(Json::integer(device));
+  devices.push(Json::integer(req.device));
   report["devices"] = devices;
-  report["state_bytes"] = Json::integer(info.state_bytes);
-  report["states_sha256"] = Json::str(mgbfs::host::sha256_hex(states_text));
-  report["candidates_generated"] = Json::unsigned_(info.candidates);  // after back-edge pruning
-  report["kernel_seconds"] = Json::real(info.kernel_seconds);
+  report["state_bytes"] = Json::integer(r.info.state_bytes);
+  report["states_sha256"] = Json::str(mgbfs::host::sha256_hex(r.states_text));
+  report["candidates_generated"] = Json::unsigned_(r.info.candidates);  // after back-edge pruning
+  report["kernel_seconds"] = Json::real(r.info.kernel_seconds);
   report["retention"] = Json::str("current sample <=1000 and previous layer only if <1000");
   report["scope"] = Json::str("single-device MI355X engine");
-  if (cache) cache->device_bytes = info.device_bytes;
-  else mgbfs_engine_destroy(engine);
-  write_file(out + "/states.json", states_text);
+  write_file(out + "/states.json", r.states_text);
   write_file(out + "/report.json", mgbfs::host::dumps(report));
   return report;
 }
@@ -633,50 +744,252 @@ int cmd_graph_count() {
   return 0;
 }
 
-int cmd_graph_info(const std::vector<std::string>& args) {
-  if (args.size() < 2) throw CliError("CLI_USAGE: mgbfs graph-info <graph.json> <devices> [shards] [capacity] [batch]");
-  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
-  std::vector<int32_t> devs;
+// Device list of graph-info: "auto" = every visible device, else a comma
+// list of distinct visible ordinals (generic_distributed_run.rs:24-28).
+std::vector<int32_t> parse_devices(const std::string& spec) {
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
-  if (args[1] == "auto") {
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw CliError("NO_VISIBLE_HIP_DEVICES");
+  std::vector<int32_t> devs;
+  if (spec == "auto") {
     for (int d = 0; d < n; ++d) devs.push_back(d);
   } else {
-    std::stringstream ss(args[1]);
+    std::stringstream ss(spec);
     std::string tok;
-    while (std::getline(ss, tok, ',')) devs.push_back(static_cast<int32_t>(parse_u64(tok, "CLI_GRAPH_DEVICE")));
+    while (std::getline(ss, tok, ',')) devs.push_back(static_cast<int32_t>(parse_u64(tok, "CLI_GRAPH_DEVICES")));
   }
-  if (devs.empty()) throw CliError("GRAPH_INFO_NO_DEVICES");
+  bool bad = devs.empty() || devs.size() > 128;
+  for (size_t i = 0; i < devs.size() && !bad; ++i)
+    bad = devs[i] >= n || std::find(devs.begin(), devs.begin() + i, devs[i]) != devs.begin() + i;
+  if (bad) throw CliError("CLI_GRAPH_DEVICE_SELECTION");
+  return devs;
+}
+
+Json inventory_entry(int32_t device) {
+  uint32_t cu = 0, ws = 0, lds = 0;
+  uint64_t tot = 0, fr = 0, hw[8] = {0};
+  if (mgbfs_hip_device_info(device, &cu, &ws, &lds, &tot, &fr)) throw CliError("GRAPH_INFO_DEVICE");
+  if (mgbfs_generic_hardware_info(device, hw)) throw CliError("GRAPH_INFO_HARDWARE");
+  Json e = Json::object();
+  e["device"] = Json::integer(device);
+  e["free_bytes"] = Json::unsigned_(fr);
+  e["total_bytes"] = Json::unsigned_(tot);
+  static const char* names[8] = {"sm_count",     "compute_major",  "compute_minor",     "l2_bytes",
+                                 "warp_size",    "threads_per_sm", "threads_per_block", "shared_bytes_per_sm"};
+  for (int i = 0; i < 8; ++i) e[names[i]] = Json::unsigned_(hw[i]);
+  return e;
+}
+
+// Admission (generic_distributed_run.rs:33-73): one rank plan per inventory
+// entry, sized against that entry's free bytes less the 2 GiB reserve (a
+// live query when budgets are 0), the smallest as `plan`, owner cut points
+// proportional to the rank capacities (none when a capacity was requested).
+// A requested capacity or batch that a budget cannot hold is refused with
+// the reference's codes.
+Json admission(const GraphDoc& graph, const Json& inventory, const std::vector<uint64_t>& budgets, uint64_t shards,
+               uint64_t requested, uint64_t batch) {
+  std::vector<uint32_t> pflat;
+  std::vector<int64_t> mflat;
+  const mgbfs_graph_desc desc = graph.desc(pflat, mflat);
+  const uint32_t world = static_cast<uint32_t>(inventory.a.size());
   Json out = Json::object();
+  out["size_profile_capability"] = Json::integer(1);
   out["graph_digest"] = Json::str(graph.digest());
-  Json dj = Json::array(), inv = Json::array();
-  for (int32_t d : devs) {
-    dj.push(Json::integer(d));
-    uint32_t cu = 0, ws = 0, lds = 0;
-    uint64_t tot = 0, fr = 0;
-    if (mgbfs_hip_device_info(d, &cu, &ws, &lds, &tot, &fr)) throw CliError("GRAPH_INFO_DEVICE");
-    Json e = Json::object();
-    e["device"] = Json::integer(d);
-    e["compute_units"] = Json::integer(cu);
-    e["wave_size"] = Json::integer(ws);
-    e["total_bytes"] = Json::unsigned_(tot);
-    e["free_bytes"] = Json::unsigned_(fr);
-    inv.push(e);
+  Json dj = Json::array(), plans = Json::array();
+  std::vector<mgbfs_engine_plan_info> infos;
+  size_t smallest = 0;
+  for (size_t i = 0; i < inventory.a.size(); ++i) {
+    const Json& entry = inventory.a[i];
+    mgbfs_engine_options opts{};
+    opts.device = static_cast<int32_t>(entry.at("device").as_int());
+    opts.capacity = requested;
+    opts.vram_budget = budgets[i];
+    opts.batch = static_cast<uint32_t>(std::min<uint64_t>(batch, 0xffffffffull));
+    dj.push(Json::integer(opts.device));
+    mgbfs_engine_plan_info p{};
+    char err[256] = {0};
+    if (mgbfs_engine_plan(&desc, &opts, &p, err, sizeof err)) throw CliError(err[0] ? err : "GRAPH_INFO_PLAN");
+    if (requested && (p.capacity != requested || p.device_bytes > p.budget_bytes))
+      throw CliError("REQUESTED_CAPACITY_EXCEEDS_ADMISSION");
+    if (batch) {
+      if (batch > p.capacity) throw CliError("REQUESTED_BATCH_EXCEEDS_ADMISSION");
+      p.batch = batch;
+      p.queue_capacity = batch * desc.generators;
+    }
+    infos.push_back(p);
+    if (p.capacity < infos[smallest].capacity) smallest = infos.size() - 1;
+  }
+  uint64_t total = 0;
+  for (const auto& p : infos) {
+    plans.push(plan_json(graph, p, world, static_cast<uint32_t>(shards)));
+    total += p.capacity;
   }
   out["devices"] = dj;
-  out["inventory"] = inv;
-  Json plan = Json::object();
-  const bool packable = graph.kind == 0 && graph.elements <= 16 &&
-                        std::all_of(graph.start.begin(), graph.start.end(), [](int64_t v) { return v >= 0 && v < 16; });
-  const bool closed = graph.inverse_closed();
-  plan["inverse_closed"] = Json::boolean(closed);
-  plan["history_layers"] = Json::integer(closed ? 3 : 0);
-  plan["packed_u64"] = Json::boolean(packable);
-  plan["backend"] = Json::str(packable && closed ? "sorted" : (packable ? "permutation_bitmap" : "generic"));
-  plan["elements"] = Json::integer(graph.elements);
+  out["inventory"] = inventory;
+  out["plan"] = plan_json(graph, infos[smallest], world, static_cast<uint32_t>(shards));
+  out["rank_plans"] = plans;
+  if (requested) {
+    out["owner_cuts"] = Json();
+  } else {
+    Json cuts = Json::array();
+    cuts.push(Json::integer(0));
+    unsigned __int128 cumulative = 0;
+    for (const auto& p : infos) {
+      cumulative += p.capacity;
+      cuts.push(Json::unsigned_(static_cast<uint64_t>(((cumulative << 32) + total - 1) / total)));
+    }
+    out["owner_cuts"] = cuts;
+  }
+  out["additional_reserve_bytes"] = Json::unsigned_(kReserveBytes);
+  out["reserve_policy"] = Json::str("ABSOLUTE_BYTES_COLD_2GIB_WARM_256MIB");
+  out["profile_status"] = Json::str("MEMORY_ADMITTED_NOT_THROUGHPUT_TUNED");
+  return out;
+}
+
+// Optional [capacity|auto] [batch] arguments of the admission commands.
+void parse_capacity_batch(const std::vector<std::string>& args, size_t at, uint64_t& requested, uint64_t& batch) {
+  requested = args.size() > at && args[at] != "auto" ? parse_u64(args[at], "CLI_GRAPH_CAPACITY") : 0;
+  batch = args.size() > at + 1 ? parse_u64(args[at + 1], "CLI_GRAPH_BATCH") : 0;
+  if (args.size() > at && args[at] != "auto" && requested == 0) throw CliError("CLI_GRAPH_CAPACITY");
+  if (args.size() > at + 1 && batch == 0) throw CliError("CLI_GRAPH_BATCH");
+}
+
+uint64_t parse_shards(const std::string& s) {
+  const uint64_t shards = parse_u64(s, "CLI_GRAPH_SHARDS");
+  if (shards == 0 || shards > 4096) throw CliError("CLI_GRAPH_SHARDS");
+  return shards;
+}
+
+// graph-info <graph.json> <devices> <shards> [capacity|auto] [batch]: the
+// admission of the selected local devices, from live free memory.
+int cmd_graph_info(const std::vector<std::string>& args) {
+  if (args.size() < 3 || args.size() > 5) throw CliError("CLI_GRAPH_INFO_ARGUMENTS");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  const uint64_t shards = parse_shards(args[2]);
+  const std::vector<int32_t> devs = parse_devices(args[1]);
+  uint64_t requested = 0, batch = 0;
+  parse_capacity_batch(args, 3, requested, batch);
+  Json inv = Json::array();
+  for (int32_t d : devs) inv.push(inventory_entry(d));
+  std::vector<uint64_t> budgets(devs.size(), 0);  // live hipMemGetInfo per device
+  std::cout << mgbfs::host::dumps(admission(graph, inv, budgets, shards, requested, batch)) << std::endl;
+  return 0;
+}
+
+// graph-plan <graph.json> <inventory.json> <shards> [capacity|auto] [batch]:
+// the same admission from an inventory list written by graph-info or by
+// another host (generic_distributed_run.rs:91-96); no device is queried.
+int cmd_graph_plan(const std::vector<std::string>& args) {
+  if (args.size() < 3 || args.size() > 5) throw CliError("CLI_GRAPH_GLOBAL_ARGUMENTS");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  const Json inventory = mgbfs::host::JsonParser(read_file(args[1])).parse();
+  const uint64_t shards = parse_shards(args[2]);
+  uint64_t requested = 0, batch = 0;
+  parse_capacity_batch(args, 3, requested, batch);
+  if (inventory.kind != Json::Arr || inventory.a.empty() || inventory.a.size() > 128)
+    throw CliError("GRAPH_GLOBAL_INVENTORY_SHAPE");
+  std::vector<uint64_t> budgets;
+  for (const Json& e : inventory.a) {
+    if (e.kind != Json::Obj || !e.has("device") || !e.at("device").is_int() || e.at("device").as_int() < 0)
+      throw CliError("GRAPH_GLOBAL_DEVICE");
+    if (!e.has("free_bytes") || !e.at("free_bytes").is_int()) throw CliError("GRAPH_GLOBAL_FREE_BYTES");
+    if (!e.has("total_bytes") || !e.at("total_bytes").is_int()) throw CliError("GRAPH_GLOBAL_TOTAL_BYTES");
+    const uint64_t fr = static_cast<uint64_t>(e.at("free_bytes").as_int());
+    const uint64_t tot = static_cast<uint64_t>(e.at("total_bytes").as_int());
+    if (fr == 0 || fr > tot) throw CliError("GRAPH_GLOBAL_MEMORY_RANGE");
+    if (fr <= kReserveBytes) throw CliError("GENERIC_DISTRIBUTED_HEADROOM");
+    budgets.push_back(fr - kReserveBytes);
+  }
+  std::cout << mgbfs::host::dumps(admission(graph, inventory, budgets, shards, requested, batch)) << std::endl;
+  return 0;
+}
+
+// graph-local-plan <graph.json> <device> <world> <shards> <capacity|auto> <batch>:
+// this device's plan and inventory figures only (generic_distributed_run.rs:75-90).
+int cmd_graph_local_plan(const std::vector<std::string>& args) {
+  if (args.size() != 6) throw CliError("CLI_GRAPH_LOCAL_PLAN_ARGUMENTS");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  const int32_t device = static_cast<int32_t>(parse_u64(args[1], "CLI_GRAPH_DEVICE"));
+  const uint64_t world = parse_u64(args[2], "CLI_GRAPH_WORLD");
+  const uint64_t shards = parse_shards(args[3]);
+  if (world == 0 || world > 128) throw CliError("CLI_GRAPH_LOCAL_PLAN_GEOMETRY");
+  uint64_t requested = 0, batch = 0;
+  parse_capacity_batch(args, 4, requested, batch);
+  if (batch == 0) throw CliError("CLI_GRAPH_BATCH");
+  Json inv = Json::array();
+  inv.push(inventory_entry(device));
+  const Json adm = admission(graph, inv, std::vector<uint64_t>(1, 0), shards, requested, batch);
+  Json out = inv.a[0];
+  out["size_profile_capability"] = Json::integer(1);
+  out["graph_digest"] = Json::str(graph.digest());
+  Json plan = adm.at("plan");
+  plan["world"] = Json::unsigned_(world);
   out["plan"] = plan;
-  out["additional_reserve_bytes"] = Json::unsigned_(2ull << 30);
   std::cout << mgbfs::host::dumps(out) << std::endl;
+  return 0;
+}
+
+// Rank worker of the reference launcher (generic_distributed_run.rs:109-232)
+// for a world of one device: reads the admission written by graph-info,
+// runs the BFS on devices[rank] with the rank plan's backend, capacity and
+// batch, and writes <out>/report.json (with the rank plan echoed verbatim)
+// and <out>/states.json.  Worlds above one need the owner exchange, which
+// this build does not provide.
+int cmd_graph_rank(const std::vector<std::string>& args) {
+  if (args.size() != 6) throw CliError("CLI_GRAPH_RANK_ARGUMENTS");
+  const Json launch = mgbfs::host::JsonParser(read_file(args[1])).parse();
+  const uint64_t rank = parse_u64(args[2], "CLI_GRAPH_RANK");
+  GraphDoc graph(mgbfs::host::JsonParser(read_file(args[0])).parse());
+  if (launch.kind != Json::Obj || !launch.has("graph_digest") || !launch.has("devices") || !launch.has("plan") ||
+      launch.at("devices").kind != Json::Arr || launch.at("plan").kind != Json::Obj)
+    throw CliError("GRAPH_LAUNCH_IDENTITY_OR_GEOMETRY");
+  const Json& devices = launch.at("devices");
+  const Json& base = launch.at("plan");
+  const uint64_t world = base.has("world") && base.at("world").is_int() ? static_cast<uint64_t>(base.at("world").as_int()) : 0;
+  if (launch.at("graph_digest").kind != Json::Str || launch.at("graph_digest").s != graph.digest() ||
+      devices.a.size() != world || rank >= world)
+    throw CliError("GRAPH_LAUNCH_IDENTITY_OR_GEOMETRY");
+  if (world != 1) throw CliError("GRAPH_RANK_WORLD_UNSUPPORTED");
+  const Json& rank_plan =
+      launch.has("rank_plans") && launch.at("rank_plans").kind == Json::Arr && !launch.at("rank_plans").a.empty()
+          ? launch.at("rank_plans").a.at(rank)
+          : base;
+  auto field = [&](const char* k) -> uint64_t {
+    if (!rank_plan.has(k) || !rank_plan.at(k).is_int() || rank_plan.at(k).as_int() < 0) throw CliError("GRAPH_RANK_PLANS");
+    return static_cast<uint64_t>(rank_plan.at(k).as_int());
+  };
+  BfsRequest req;
+  req.device = static_cast<int32_t>(devices.a[rank].as_int());
+  req.capacity = field("capacity");
+  req.batch = static_cast<uint32_t>(std::min<uint64_t>(field("batch"), 0xffffffffull));
+  req.backend = rank_plan.has("backend") && rank_plan.at("backend").kind == Json::Str ? rank_plan.at("backend").s : "auto";
+  req.seconds = parse_u64(args[5], "CLI_GRAPH_SECONDS");
+  const char* phase = std::getenv("MGBFS_JOB_PHASE_PATH");
+  req.phase_path = phase ? phase : "";
+  const std::string out = args[4];
+  if (path_exists(out)) throw CliError("CLI_GRAPH_OUTPUT_EXISTS");
+  if (mkdir(out.c_str(), 0755)) throw CliError("CLI_GRAPH_OUTPUT_CREATE");
+  const BfsOutcome r = run_bfs(graph, req, nullptr, "", nullptr, "current_sample_global_limit");
+  Json report = Json::object();
+  report["schema"] = Json::integer(2);
+  report["rank"] = Json::unsigned_(rank);
+  report["world"] = Json::unsigned_(world);
+  report["device"] = Json::integer(req.device);
+  report["status"] = Json::str(r.status);
+  report["reason"] = Json::str(r.reason);
+  report["graph_digest"] = Json::str(graph.digest());
+  report["states_sha256"] = Json::str(mgbfs::host::sha256_hex(r.states_text));
+  Json seconds;
+  report["layer_sizes"] = layer_json(r, seconds);
+  report["layer_seconds"] = seconds;
+  report["bfs_seconds"] = Json::real(r.bfs_seconds);
+  report["setup_seconds"] = Json::real(r.setup_seconds);
+  report["source_retries"] = Json::unsigned_(r.info.retries);
+  report["plan"] = rank_plan;
+  report["backend"] = Json::str(r.backend);
+  report["candidates_generated"] = Json::unsigned_(r.info.candidates);
+  write_file(out + "/states.json", r.states_text);
+  write_file(out + "/report.json", mgbfs::host::dumps(report));
+  write_job_phase(req.phase_path, "done");
   return 0;
 }
 
@@ -701,7 +1014,8 @@ int cmd_key_info() {
   for (const char* x : {"sorted", "permutation_bitmap", "generic"}) b.push(Json::str(x));
   out["backends"] = b;
   Json c = Json::array();
-  for (const char* x : {"graph", "graph-info", "graph-count", "graph-digest", "key-info", "version"})
+  for (const char* x : {"graph", "graph-info", "graph-plan", "graph-local-plan", "graph-rank", "graph-session",
+                        "graph-count", "graph-digest", "key-info", "version"})
     c.push(Json::str(x));
   out["commands"] = c;
   out["max_packed_degree"] = Json::integer(16);
@@ -717,12 +1031,17 @@ int main(int argc, char** argv) {
   std::vector<std::string> args(argv + 1, argv + argc);
   try {
     if (args.empty())
-      throw CliError("CLI_USAGE: mgbfs <graph|graph-session|graph-info|graph-count|graph-digest|key-info|version> ...");
+      throw CliError(
+          "CLI_USAGE: mgbfs <graph|graph-session|graph-info|graph-plan|graph-local-plan|graph-rank|graph-count|"
+          "graph-digest|key-info|version> ...");
     const std::string cmd = args[0];
     std::vector<std::string> rest(args.begin() + 1, args.end());
     if (cmd == "graph") return cmd_graph(rest);
     if (cmd == "graph-session") return cmd_graph_session(rest);
     if (cmd == "graph-info") return cmd_graph_info(rest);
+    if (cmd == "graph-rank") return cmd_graph_rank(rest);
+    if (cmd == "graph-plan") return cmd_graph_plan(rest);
+    if (cmd == "graph-local-plan") return cmd_graph_local_plan(rest);
     if (cmd == "graph-count") return cmd_graph_count();
     if (cmd == "graph-digest") return cmd_graph_digest(rest);
     if (cmd == "key-info") return cmd_key_info();

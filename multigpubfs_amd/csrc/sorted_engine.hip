@@ -46,15 +46,7 @@ class SortedEngine final : public Engine {
     layout_.km = make_keymix(4 * n_);
     layout_.tagbits = 64 - 4 * n_;
     layout_.tagmask = layout_.tagbits ? ((1ull << layout_.tagbits) - 1) : 0;
-    if (!fill_back_edge_masks(gens_)) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
-    // Back-edge masks use tag bits 0..G-1; bit tagbits-1 is the in-bank
-    // mark and bit tagbits-2 stays clear, so no entry is all ones.
-    layout_.prune = gens_.count + 2 <= layout_.tagbits && !(flags_ & MGBFS_FLAG_TEST_NO_PRUNE);
-    if (!layout_.prune)
-      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_mask[a] = 0;
-    history_layers = layout_.prune ? 2 : 3;
-    margin_ = fine_margin(gens_.count);
-    start_key_ = g.start_key();
+    set_generators(g);
     // One workgroup per sub-bucket: persistent grids of 2 or 8 workgroups
     // per CU, prefetching the next sub-bucket's bounds, measured slower
     // (127 and 122 vs 120 ms per S13 BFS).
@@ -233,6 +225,14 @@ class SortedEngine final : public Engine {
     }
   }
 
+  // Another inverse-closed graph of the same degree reuses every bank and slab.
+  void rebind(const Graph& g) override {
+    if (g.elements != n_ || !g.packable() || !g.inverse_closed) throw std::runtime_error("ENGINE_REBIND_SHAPE");
+    gens_ = g.packed_generators();
+    set_generators(g);
+    reset();
+  }
+
   void layer_keys(std::vector<uint64_t>& out) override { read_bank(cur_, count_[cur_], out); }
 
   void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) override {
@@ -265,6 +265,18 @@ class SortedEngine final : public Engine {
 
  private:
   uint32_t* error_word() { return reinterpret_cast<uint32_t*>(control_.as<uint64_t>() + 1); }
+  // Back-edge masks of gens_ (tag bits 0..G-1; bit tagbits-1 is the in-bank
+  // mark and bit tagbits-2 stays clear, so no entry is all ones), the
+  // fine-bucket margin and the start state.
+  void set_generators(const Graph& g) {
+    if (!fill_back_edge_masks(gens_)) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
+    layout_.prune = gens_.count + 2 <= layout_.tagbits && !(flags_ & MGBFS_FLAG_TEST_NO_PRUNE);
+    if (!layout_.prune)
+      for (uint32_t a = 0; a < gens_.count; ++a) gens_.child_mask[a] = 0;
+    history_layers = layout_.prune ? 2 : 3;
+    margin_ = fine_margin(gens_.count);
+    start_key_ = g.start_key();
+  }
   // Compactions run unwaited; each is bracketed by one of two event pairs,
   // read (and charged to phase 3) when the pair is reused or on settle().
   // A pair is reused two compactions later, after the host has waited on

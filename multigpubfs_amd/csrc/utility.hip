@@ -72,3 +72,19 @@ extern "C" int mgbfs_hip_device_info(int32_t device, uint32_t* compute_units, ui
   if (free_bytes) *free_bytes = fr;
   return 0;
 }
+
+extern "C" int mgbfs_generic_hardware_info(int device, uint64_t* values) {
+  if (!values) return hipErrorInvalidValue;
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return e;
+  values[0] = static_cast<uint64_t>(prop.multiProcessorCount);
+  values[1] = static_cast<uint64_t>(prop.major);
+  values[2] = static_cast<uint64_t>(prop.minor);
+  values[3] = static_cast<uint64_t>(prop.l2CacheSize);
+  values[4] = static_cast<uint64_t>(prop.warpSize);
+  values[5] = static_cast<uint64_t>(prop.maxThreadsPerMultiProcessor);
+  values[6] = static_cast<uint64_t>(prop.maxThreadsPerBlock);
+  values[7] = static_cast<uint64_t>(prop.maxSharedMemoryPerMultiProcessor);
+  return 0;
+}

@@ -146,7 +146,7 @@ def test_foreign_replacement_is_detected(fake_cayleypy):
     FakeCayleyGraph.bfs = lambda self, **kw: "other"
     with pytest.raises(RuntimeError):
         api.disable_native()
-    FakeCayleyGraph.bfs = api._INSTALLED
+    FakeCayleyGraph.bfs = api._HOOK.installed
     api.disable_native()
 
 
@@ -186,12 +186,41 @@ def test_incomplete_has_no_diameter(fake_cayleypy):
 
 
 def test_options_validation():
-    with pytest.raises(ValueError):
-        api.NativeOptions(memory_preset="fast")
-    with pytest.raises(ValueError):
-        api.NativeOptions(segment_rows=0)
-    with pytest.raises(ValueError):
-        api.NativeOptions(memory_backend="tape")
+    # cayleypy_backend.py:24-36 of the reference: same fields and codes.
+    for kwargs, code in [(dict(memory_preset="fast"), "INVALID_MEMORY_PRESET"),
+                         (dict(segment_rows=0), "INVALID_SEGMENT_ROWS"),
+                         (dict(segment_rows=(1 << 32) + 1), "INVALID_SEGMENT_ROWS"),
+                         (dict(memory_backend="tape"), "INVALID_MEMORY_BACKEND"),
+                         (dict(segment_strategy="fast"), "INVALID_SEGMENT_STRATEGY"),
+                         (dict(bitmap_owner_page=3), "INVALID_BITMAP_OWNER_PAGE"),
+                         (dict(bitmap_owner_page=1 << 21, memory_backend="permutation_bitmap"),
+                          "INVALID_BITMAP_OWNER_PAGE"),
+                         (dict(bitmap_owner_page=True), "INVALID_BITMAP_OWNER_PAGE"),
+                         (dict(bitmap_owner_page=64), "BITMAP_OWNER_PAGE_REQUIRES_BITMAP")]:
+        with pytest.raises(ValueError, match=code):
+            api.NativeOptions(**kwargs)
+    ok = api.NativeOptions(bitmap_owner_page=64, memory_backend="permutation_bitmap", segment_strategy="warp")
+    assert ok.bitmap_owner_page == 64 and api.NativeOptions(bitmap_owner_page=0).bitmap_owner_page == 0
+
+
+def test_segment_strategy_default_follows_environment(monkeypatch):
+    monkeypatch.setenv("MGBFS_SEGMENT_STRATEGY", "batch")
+    assert api.NativeOptions().segment_strategy == "batch"
+    monkeypatch.setenv("MGBFS_SEGMENT_STRATEGY", "bogus")
+    with pytest.raises(ValueError, match="INVALID_SEGMENT_STRATEGY"):
+        api.NativeOptions()
+
+
+def test_options_reach_run_graph(fake_cayleypy):
+    calls = []
+    options = api.NativeOptions(devices=(0,), max_seconds=7, memory_backend="permutation_bitmap",
+                                bitmap_owner_page=1024, segment_strategy="atomic", autotune=False)
+    with patch.object(api, "run_graph", side_effect=fake_native(calls)):
+        api.bfs(FakeCayleyGraph(), native_options=options, output=fake_cayleypy.root / "opts")
+    kwargs = calls[0][1]
+    assert kwargs["devices"] == [0] and kwargs["max_seconds"] == 7 and kwargs["autotune"] is False
+    assert kwargs["bitmap_owner_page"] == 1024 and kwargs["segment_strategy"] == "atomic"
+    assert kwargs["memory_backend"] == "permutation_bitmap"
 
 
 def test_run_graph_argument_validation(tmp_path):
@@ -310,3 +339,48 @@ def test_serving_job_stream_mix_and_rate():
         else:
             p = gd.action["generators"][0]["modulo"]
             assert 101 <= p <= 521 and all(p % d for d in range(2, p))
+
+
+def test_graph_plan_from_inventory_needs_no_device(tmp_path):
+    # graph-plan sizes rank plans from a supplied inventory (free bytes less
+    # the 2 GiB reserve), so it runs on a host without a GPU.
+    gib = 1 << 30
+    inventory = [{"device": 0, "free_bytes": 100 * gib, "total_bytes": 288 * gib},
+                 {"device": 1, "free_bytes": 50 * gib, "total_bytes": 288 * gib}]
+    inv = tmp_path / "inventory.json"
+    inv.write_text(json.dumps(inventory))
+    p = 13
+    sl2 = GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], p), ([1, 0, 1, 1], p), ([1, p - 1, 0, 1], p),
+                                        ([1, 0, p - 1, 1], p)], [1, 0, 0, 1])
+    directed = GraphDefinition.permutation([[1, 2, 3, 4, 5, 6, 0], [1, 0, 3, 2, 5, 4, 6]], list(range(7)))
+    for g, backend, history in ((lrx(13), "sorted", 3), (directed, "permutation_bitmap", 1), (sl2, "generic", 3)):
+        path = tmp_path / f"{backend}.json"
+        path.write_text(g.to_json())
+        proc = cli("graph-plan", path, inv, 4)
+        assert proc.returncode == 0, proc.stderr
+        doc = json.loads(proc.stdout)
+        assert doc["graph_digest"] == g.digest() and doc["devices"] == [0, 1]
+        plans = doc["rank_plans"]
+        assert [q["backend"] for q in plans] == [backend, backend]
+        assert all(q["history_layers"] == history and q["world"] == 2 and q["shards"] == 4 for q in plans)
+        for q, item in zip(plans, inventory):
+            assert q["device_bytes"] <= item["free_bytes"] - 2 * gib
+        assert doc["plan"] == min(plans, key=lambda q: q["capacity"])
+        cuts = doc["owner_cuts"]
+        assert cuts[0] == 0 and cuts[-1] == 1 << 32 and cuts == sorted(cuts) and len(cuts) == 3
+    # Generic arena plan: the reference formula elements*capacity*state_bytes*2
+    # + capacity*8 + next_pow2(2 capacity)*8 + 24 (generic_memory.rs:36-41),
+    # plus the generator tables; capacity capped by the bound p^4.
+    q = doc["plan"]
+    assert q["capacity"] == p ** 4 and q["table_slots"] == 1 << 16 and q["state_bytes"] == 8
+    tables = 4 * 4 * 8 + 4 * 4
+    assert q["device_bytes"] == 4 * p ** 4 * 8 * 2 + p ** 4 * 8 + (1 << 16) * 8 + 24 + tables
+    assert q["generator_bytes"] == 4 * (4 * 8 + 4)
+    bad = tmp_path / "bad.json"
+    for entry, code in (({"device": 0, "free_bytes": 3 * gib, "total_bytes": 2 * gib}, "GRAPH_GLOBAL_MEMORY_RANGE"),
+                        ({"device": 0, "free_bytes": gib, "total_bytes": 2 * gib}, "GENERIC_DISTRIBUTED_HEADROOM"),
+                        ({"free_bytes": gib, "total_bytes": 2 * gib}, "GRAPH_GLOBAL_DEVICE")):
+        bad.write_text(json.dumps([entry]))
+        assert last_error(cli("graph-plan", tmp_path / "generic.json", bad, 1))["error"] == code
+    assert last_error(cli("graph-plan", tmp_path / "generic.json", inv, 1, "1000", "2000"))["error"] == \
+        "REQUESTED_BATCH_EXCEEDS_ADMISSION"

@@ -111,3 +111,148 @@ def test_archive_of_resource_stop_commits_finished_layers(tmp_path):
     report = run_graph(lrx(8), tmp_path / "r", device=0, capacity=50, memory_backend="generic", archive=True)
     assert report["status"] == "INCOMPLETE"
     assert report["archive"]["depths"] == len(report["layer_sizes"])
+
+
+# ------------------------------------------------------------------ admission + rank protocol
+
+import subprocess  # noqa: E402
+
+from multigpubfs_amd._native import CLI_PATH  # noqa: E402
+
+INVENTORY_KEYS = {"device", "free_bytes", "total_bytes", "sm_count", "compute_major", "compute_minor", "l2_bytes",
+                  "warp_size", "threads_per_sm", "threads_per_block", "shared_bytes_per_sm"}
+PLAN_KEYS = {"generator_backend", "history_algorithm", "owner_lanes", "sorted_owner_bytes",
+             "driver_graph_reserve_bytes", "parent_transport", "sort_candidates", "history_layers", "state_bytes",
+             "elements", "world", "shards", "capacity", "batch", "table_layout", "queue_capacity", "table_slots",
+             "generator_bytes", "device_bytes"}
+
+
+def mgbfs(*args, check=True):
+    proc = subprocess.run([str(CLI_PATH), *map(str, args)], capture_output=True, text=True, timeout=300)
+    if check and proc.returncode:
+        raise AssertionError(proc.stderr)
+    return proc
+
+
+def error_code(proc):
+    assert proc.returncode != 0
+    return json.loads(proc.stderr.strip().splitlines()[-1])["error"]
+
+
+def directed_graph():
+    # Two random directed permutations on S7 (no inverses): full history.
+    return GraphDefinition.permutation([[1, 2, 3, 4, 5, 6, 0], [1, 0, 3, 2, 5, 4, 6]], list(range(7)))
+
+
+def sl2(p):
+    return GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], p), ([1, 0, 1, 1], p), ([1, p - 1, 0, 1], p),
+                                         ([1, 0, p - 1, 1], p)], [1, 0, 0, 1])
+
+
+@pytest.mark.parametrize("name,graph,history,backend", [
+    ("lrx8", lrx(8), 3, "sorted"), ("directed7", directed_graph(), 1, "permutation_bitmap"),
+    ("sl2_13", sl2(13), 3, "generic")])
+def test_graph_info_admission_contract(tmp_path, name, graph, history, backend):
+    # generic_distributed_run.rs:23-73: inventory per device, a rank plan per
+    # device, the smallest as `plan`, owner cut points, reserve policy.
+    path = tmp_path / "graph.json"
+    path.write_text(graph.to_json())
+    info = json.loads(mgbfs("graph-info", path, "0", "1").stdout)
+    assert info["graph_digest"] == graph.digest() and info["devices"] == [0]
+    assert set(info["inventory"][0]) >= INVENTORY_KEYS
+    inv = info["inventory"][0]
+    assert inv["warp_size"] == 64 and (inv["compute_major"], inv["compute_minor"]) == (9, 5)
+    plan = info["plan"]
+    assert set(plan) >= PLAN_KEYS and info["rank_plans"] == [plan]
+    assert plan["history_layers"] == history and plan["backend"] == backend
+    assert plan["world"] == 1 and plan["shards"] == 1 and plan["elements"] == len(graph.to_dict()["start"])
+    assert 0 < plan["batch"] <= plan["capacity"]
+    assert plan["queue_capacity"] == plan["batch"] * len(graph.to_dict()["action"]["generators"])
+    assert 0 < plan["device_bytes"] <= inv["free_bytes"]
+    if backend == "generic":
+        assert plan["table_slots"] >= 2 * plan["capacity"] and plan["history_algorithm"] == "HASH"
+        # Capped by the state-space bound p^(rows*cols) of a single modulus
+        # (generic_memory.rs:82-108).
+        assert plan["capacity"] == plan["state_bound"] == 13 ** 4
+    assert info["owner_cuts"] == [0, 1 << 32]
+    assert info["additional_reserve_bytes"] == 2 << 30
+
+
+def test_graph_info_requested_capacity_and_batch(tmp_path):
+    path = tmp_path / "graph.json"
+    path.write_text(sl2(13).to_json())
+    info = json.loads(mgbfs("graph-info", path, "auto", "2", "5000", "100").stdout)
+    assert info["plan"]["capacity"] == 5000 and info["plan"]["batch"] == 100 and info["plan"]["shards"] == 2
+    assert info["owner_cuts"] is None
+    assert error_code(mgbfs("graph-info", path, "0", "1", "auto", "0", check=False)) == "CLI_GRAPH_BATCH"
+    assert error_code(mgbfs("graph-info", path, "0", "1", "100", "101", check=False)) == \
+        "REQUESTED_BATCH_EXCEEDS_ADMISSION"
+    assert error_code(mgbfs("graph-info", path, "0", "1", str(1 << 40), check=False)) == \
+        "REQUESTED_CAPACITY_EXCEEDS_ADMISSION"
+    assert error_code(mgbfs("graph-info", path, "0,0", "1", check=False)) == "CLI_GRAPH_DEVICE_SELECTION"
+    assert error_code(mgbfs("graph-info", path, "0", check=False)) == "CLI_GRAPH_INFO_ARGUMENTS"
+
+
+def reference_launch(graph, output, native=CLI_PATH, max_seconds=600):
+    """The one-device path of the reference launcher (multigpubfs/launch.py:
+    311-413) as a foreign caller: graph-info admission, then either the
+    direct `graph` command (history_layers 1, HASH) or one graph-rank worker
+    and the launcher's receipt checks and aggregation."""
+    output.parent.mkdir(parents=True, exist_ok=True)
+    definition = output.parent / f"{output.name}-graph.json"
+    definition.write_text(graph.to_json())
+    admission = json.loads(mgbfs("graph-info", definition, "auto", "1").stdout)
+    devices, plan = admission["devices"], admission["plan"]
+    assert admission["graph_digest"] == graph.digest()
+    if plan.get("history_layers", 1) == 1 and plan.get("history_algorithm", "HASH") == "HASH" and len(devices) == 1:
+        mgbfs("graph", definition, output, "--device", devices[0], "--seconds", max_seconds)
+        return "direct", json.loads((output / "report.json").read_text())
+    output.mkdir()
+    (output / "launch.json").write_text(json.dumps(admission))
+    env_phase = output / "rank-0.phase.json"
+    proc = subprocess.run([str(native), "graph-rank", str(definition), str(output / "launch.json"), "0",
+                           str(output.parent / "nccl-id"), str(output / "rank-0"), str(max_seconds)],
+                          capture_output=True, text=True, timeout=600,
+                          env=dict(__import__("os").environ, RANK="0", WORLD_SIZE="1", LOCAL_RANK=str(devices[0]),
+                                   MGBFS_JOB_PHASE_PATH=str(env_phase)))
+    assert proc.returncode == 0, proc.stderr
+    part = json.loads((output / "rank-0" / "report.json").read_text())
+    raw_states = (output / "rank-0" / "states.json").read_bytes()
+    assert hashlib.sha256(raw_states).hexdigest() == part["states_sha256"]
+    assert part["graph_digest"] == graph.digest() and part["status"] in ("COMPLETE", "INCOMPLETE")
+    # launch.py:399-400: rank geometry must match the admission.
+    assert part["rank"] == 0 and part["world"] == 1 and part["device"] == devices[0]
+    assert part["plan"] == admission.get("rank_plans", [plan])[0]
+    snap = json.loads(raw_states)
+    assert len(snap["current"]) <= 1000 and len(snap["previous_small"]) < 1000
+    phase = json.loads(env_phase.read_text())
+    assert phase["schema"] == 1 and phase["phase"] == "done" and "bfs" in phase["entered"]
+    return "rank", part
+
+
+@pytest.mark.parametrize("name,graph,route", [("lrx9", lrx(9), "rank"), ("directed7", directed_graph(), "rank"),
+                                              ("sl2_11", sl2(11), "rank"),
+                                              ("directed_matrix", GraphDefinition.matrix(1, 1, [([2], 5), ([3], 7)],
+                                                                                         [4]), "direct")])
+def test_reference_launcher_protocol_on_one_device(tmp_path, name, graph, route):
+    taken, report = reference_launch(graph, tmp_path / "out")
+    assert taken == route
+    d = graph.to_dict()
+    bound = math.prod(range(1, 10)) if name == "lrx9" else 100000
+    expected = [len(x) for x in O.c_exact_layers(d, bound)]
+    assert report["layer_sizes"] == expected and report["status"] == "COMPLETE"
+
+
+def test_graph_rank_refuses_worlds_above_one(tmp_path):
+    graph = lrx(5)
+    path = tmp_path / "graph.json"
+    path.write_text(graph.to_json())
+    plan = {"world": 2, "capacity": 120, "batch": 120, "backend": "sorted"}
+    launch = {"graph_digest": graph.digest(), "devices": [0, 1], "plan": plan, "rank_plans": [plan, plan]}
+    (tmp_path / "launch.json").write_text(json.dumps(launch))
+    proc = mgbfs("graph-rank", path, tmp_path / "launch.json", "0", tmp_path / "id", tmp_path / "r0", "0", check=False)
+    assert error_code(proc) == "GRAPH_RANK_WORLD_UNSUPPORTED"
+    launch["graph_digest"] = "0" * 64
+    (tmp_path / "launch.json").write_text(json.dumps(launch))
+    proc = mgbfs("graph-rank", path, tmp_path / "launch.json", "0", tmp_path / "id", tmp_path / "r1", "0", check=False)
+    assert error_code(proc) == "GRAPH_LAUNCH_IDENTITY_OR_GEOMETRY"

@@ -300,6 +300,25 @@ def test_reset_repeats_identical_tables(backend):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
+def test_rebind_runs_other_graphs_of_the_same_degree(backend):
+    # One engine, several generator sets of degree 9: each run equals a
+    # fresh engine's and the oracle's table; device memory is kept.
+    rng = random.Random(31)
+    graphs = [lrx(9)] + [GraphDefinition.permutation(random_generators(rng, 9, k, True), list(range(9)))
+                         for k in (1, 2, 3)]
+    with NativeEngine(graphs[0], backend=backend) as e:
+        before = e.info()["device_bytes"]
+        for g in graphs:
+            e.rebind(g)
+            sizes, _ = run_all_layers(e)
+            gens = g.to_dict()["action"]["generators"]
+            assert sizes == O.packed_bfs(9, gens)["layers"]
+        assert e.info()["device_bytes"] >= before
+        with pytest.raises(RuntimeError):
+            e.rebind(lrx(8))  # another degree
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
 def test_capacity_limit_stops_with_resource(backend):
     # A capacity below the widest layer must stop as RESOURCE, never silently.
     with NativeEngine(lrx(8), backend=backend, capacity=100) as e:
