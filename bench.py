@@ -25,29 +25,42 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # Calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE per kernel for one full BFS,
 # committed beside the text summaries (profiles/*_s13_lrx.md).
-TRAFFIC_FILES = {"sorted": ROOT / "profiles" / "sorted_engine_s13_lrx.json",
-                 "bitmap": ROOT / "profiles" / "bitmap_expand_kernel_s13_lrx.json"}
+TRAFFIC_FILES = {"sorted": ROOT / "profiles" / "sorted_engine_s13_lrx.json"}
 REFERENCE_EDGES_PER_S = 18681062400 / 11.716894  # 8x H200 median, docs/validation/lrx13-eight-h200.md:26
 
 
+def host_cores() -> tuple[int, int]:
+    """(threads this job may use, processors the host reports).  A GPU job's
+    CPU share is the processors it may run on, capped by the thread count
+    the job environment sets (OMP_NUM_THREADS, 16 per GPU on the MI355X
+    boxes), whatever os.cpu_count() reports for the whole machine."""
+    nproc = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    limit = os.environ.get("OMP_NUM_THREADS")
+    if limit and limit.isdigit() and int(limit) > 0:
+        usable = min(usable, int(limit))
+    return max(1, usable), nproc
+
+
 def cpu_baseline(n: int, seconds_target: float) -> dict:
-    """The oracle's packed Lehmer-bitmap BFS on this host's cores, on a
+    """The oracle's packed Lehmer-bitmap BFS on this job's host cores, on a
     bounded prefix of the same S_n LRX BFS (stops after the layer at which
     the edge budget is reached)."""
     from oracle import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
-    budget = int(seconds_target * 150e6)  # ~150 M edges/s on 16 cores
+    cores, nproc = host_cores()
+    budget = int(seconds_target * 10e6 * cores)  # ~10 M edges/s per thread
     t0 = time.perf_counter()
     r = O.packed_bfs(n, threads=cores, stop_after_edges=budget)
     dt = time.perf_counter() - t0
-    return {"value": r["edges"] / dt, "unit": "edges/s", "cores": cores, "kind": "port",
-            "sample": f"S{n} LRX BFS prefix: first {len(r['layers'])} layers, {r['edges']} edges expanded "
-                      f"in {dt:.2f} s (oracle/mgbfs_oracle.c orc_packed_bfs, {cores} threads)"}
+    return {"value": r["edges"] / dt, "unit": "edges/s", "cores": cores, "kind": "port", "host_nproc": nproc,
+            "sample": f"S{n} LRX BFS prefix: first {len(r['layers'])} of {79 if n == 13 else '?'} layers, "
+                      f"{r['edges']} edges expanded in {dt:.2f} s (oracle/mgbfs_oracle.c orc_packed_bfs, "
+                      f"{cores} threads: the job's CPU share of a {nproc}-processor host)"}
 
 
 def sweep_section() -> dict:
     """256 S9/S10 instances: batched (one hipGraph layer loop) vs one at a time."""
-    from multigpubfs_amd.sweep import run_batched, run_sequential
+    from multigpubfs_amd.sweep import run_batched, run_sequential, run_sequential_reusing
     golden = json.loads((ROOT / "tests" / "golden" / "sweep256.json").read_text())["instances"]
     inst = [x["graph"] for x in golden]
     from multigpubfs_amd.autotune import choose_profile
@@ -57,27 +70,87 @@ def sweep_section() -> dict:
     run_batched(inst[:8], profile=profile)  # warm the kernels and the allocator
     batched, tb, info = run_batched(inst, profile=profile)
     seq, ts = run_sequential(inst)
+    reuse, tr = run_sequential_reusing(inst)
     want = [x["layer_sizes"] for x in golden]
-    return {"instances": len(inst), "batched_s": tb, "sequential_s": ts,
+    return {"instances": len(inst), "batched_s": tb, "sequential_s": ts, "sequential_reusing_s": tr,
             "batched_bfs_per_s": len(inst) / tb, "sequential_bfs_per_s": len(inst) / ts,
-            "speedup": ts / tb, "identical_tables": batched == seq, "golden_exact": batched == want,
+            "speedup": ts / tb, "speedup_vs_reusing": tr / tb,
+            "sequential_legs": {"create_per_instance": "plain engine created, run and destroyed per instance",
+                                "reusing": "plain engine per distinct degree, kept; one device allocation "
+                                           "reused by every instance of that degree"},
+            "identical_tables": batched == seq == reuse, "golden_exact": batched == want,
             "states": sum(sum(w) for w in want), "autotune_seconds": tune_s,
             "profile": {k: profile[k] for k in ("grid_per_cu", "layers_per_graph", "visited")}}
 
 
+def serving_checks(jobs, results) -> dict:
+    """Every job's table from the serving loop against an independent one:
+    permutation jobs of degree <= 10 against the oracle's committed tables
+    (tests/golden/serve2000_small.json, same seeded stream), degree 11 and 12
+    against the plain engine run alone on the sorted backend, and SL(2, Z_p)
+    jobs against |SL(2,p)| = p(p^2 - 1) and the plain engine's generic
+    backend (one run per distinct p)."""
+    from multigpubfs_amd import GraphDefinition, NativeEngine
+    golden = json.loads((ROOT / "tests" / "golden" / "serve2000_small.json").read_text())
+    small = golden["tables"] if len(jobs) == golden["count"] else {}
+    plain_sl2 = {}
+    perm = {"jobs": 0, "mismatches": 0, "vs_golden": 0, "vs_plain_engine": 0, "max_degree": 0}
+    sl2 = {"jobs": 0, "mismatches": 0, "max_p": 0}
+    t0 = time.perf_counter()
+    for done, job in enumerate(jobs):
+        if done % 250 == 0:
+            print(f"[bench] serving checks: {done}/{len(jobs)} jobs, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+        got = results[job["id"]]["layer_sizes"]
+        graph = job["graph"]
+        a = graph["action"]
+        if a["kind"] == "permutation":
+            perm["jobs"] += 1
+            perm["max_degree"] = max(perm["max_degree"], a["degree"])
+            if str(job["id"]) in small:
+                want = small[str(job["id"])]
+                perm["vs_golden"] += 1
+            else:
+                with NativeEngine(GraphDefinition(graph), backend="sorted") as e:
+                    e.run()
+                    want = e.layer_sizes()
+                perm["vs_plain_engine"] += 1
+            perm["mismatches"] += got != want
+        else:
+            p = a["generators"][0]["modulo"]
+            sl2["jobs"] += 1
+            sl2["max_p"] = max(sl2["max_p"], p)
+            if p not in plain_sl2:
+                with NativeEngine(GraphDefinition(graph), backend="generic") as e:
+                    e.run()
+                    plain_sl2[p] = e.layer_sizes()
+            sl2["mismatches"] += got != plain_sl2[p] or sum(got) != p * (p * p - 1)
+    return {"perm_tables_exact": perm["mismatches"] == 0, "sl2_tables_exact": sl2["mismatches"] == 0,
+            "permutation": perm, "sl2": sl2, "distinct_sl2_primes": len(plain_sl2),
+            "check_seconds": time.perf_counter() - t0}
+
+
 def serving_section(count: int) -> dict:
-    """Poisson stream (40 jobs/s) through the continuous-batching loop."""
+    """Poisson stream (40 jobs/s) through the continuous-batching loop, every
+    table checked; then the same jobs all arriving at once (arrival far above
+    service), whose jobs/s is the loop's capacity."""
     from multigpubfs_amd.serving import make_jobs, serve
     jobs = make_jobs(count)
+    print(f"[bench] serving {count} jobs at 40 jobs/s", file=sys.stderr, flush=True)
     report = serve(jobs, vram_budget=200 * 10 ** 9)
-    sl2_exact = True
-    for job in jobs:
-        a = job["graph"]["action"]
-        if a["kind"] == "matrix":
-            p = a["generators"][0]["modulo"]
-            sl2_exact &= sum(report["results"][job["id"]]["layer_sizes"]) == p * (p * p - 1)
     out = {k: v for k, v in report.items() if k != "results"}
-    out.update(vram_budget_bytes=200 * 10 ** 9, arrival_rate_per_s=40, sl2_orders_exact=sl2_exact)
+    out.update(vram_budget_bytes=200 * 10 ** 9, arrival_rate_per_s=40)
+    out.update(serving_checks(jobs, report["results"]))
+    print(f"[bench] serving the same {count} jobs arriving at once", file=sys.stderr, flush=True)
+    burst = [dict(job, arrival=0.0) for job in jobs]
+    saturated = serve(burst, vram_budget=200 * 10 ** 9)
+    same = all(saturated["results"][j["id"]]["layer_sizes"] == report["results"][j["id"]]["layer_sizes"]
+               for j in jobs)
+    out["saturation"] = {"jobs": saturated["jobs"], "capacity_jobs_per_s": saturated["jobs_per_s"],
+                         "makespan_s": saturated["makespan_s"], "p50_latency_s": saturated["p50_latency_s"],
+                         "p99_latency_s": saturated["p99_latency_s"],
+                         "peak_concurrent_jobs": saturated["peak_concurrent_jobs"],
+                         "errors": saturated["errors"], "tables_same_as_paced_run": same}
     return out
 
 
@@ -117,6 +190,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=13)
     ap.add_argument("--backend", default="sorted")
+    ap.add_argument("--engine-flags", type=int, default=0,
+                    help="mgbfs_engine_options.flags for the timed engine (MGBFS_FLAG_* of include/mgbfs_hip.h)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 256-instance batched sweep")
@@ -146,7 +221,7 @@ def main(argv=None):
     n = args.n
     states = math.factorial(n)
     edges = 3 * states
-    engine = NativeEngine(lrx(n), backend=args.backend)
+    engine = NativeEngine(lrx(n), backend=args.backend, flags=args.engine_flags)
     print(f"[bench] engine {engine.backend_name} S{n} LRX; warmup {args.warmup}", file=sys.stderr, flush=True)
     exact = True
     for _ in range(args.warmup):
@@ -191,7 +266,7 @@ def main(argv=None):
         models = ["8F + 8C", "16C", "8C + 8F + 8N (back-edge masks: current bank only)", "16N"]
     phases = []
     for i, name in enumerate(names):
-        if phase_s[i] <= 0:
+        if phase_s[i] <= 0 or phase_b[i] == 0:  # compaction fused into unique_diff: no bytes of its own
             continue
         gbs = phase_b[i] / phase_s[i] / 1e9
         phases.append({"kernel": name, "ms_per_step": 1e3 * phase_s[i] / args.steps,
@@ -225,6 +300,7 @@ def main(argv=None):
         "dtype": "u64",
         "data": "synthetic: implicit Cayley graph (LRX generators on S_n from the identity), no dataset",
         "config": {"workload": f"S{n} LRX full exact BFS, one GPU", "backend": engine.backend_name,
+                   "engine_flags": args.engine_flags,
                    "states": states, "layers": layers, "edges_per_step": edges,
                    "peak_layer": max(sizes), "layer_table_exact": bool(exact),
                    "unique_candidates_per_step": info["unique_candidates"],
@@ -253,7 +329,10 @@ def main(argv=None):
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(result), flush=True)
-    ok = (exact and result.get("sweep", {}).get("golden_exact", True) and not result.get("serving", {}).get("errors")
+    serving = result.get("serving", {})
+    ok = (exact and result.get("sweep", {}).get("golden_exact", True) and not serving.get("errors")
+          and serving.get("perm_tables_exact", True) and serving.get("sl2_tables_exact", True)
+          and serving.get("saturation", {}).get("tables_same_as_paced_run", True)
           and result.get("s11_bitmap", {}).get("layer_table_exact", True))
     return 0 if ok else 1
 

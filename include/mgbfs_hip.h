@@ -15,7 +15,8 @@
  *    buffer is caller-owned device memory.  `error` is a caller-zeroed sticky
  *    device word; bits: 1 table full, 2 visited capacity, 4 future capacity,
  *    8 bad row reference, 32 count above bound, 64 bucket above LDS capacity,
- *    128 state outside the packed/rank domain.  On any error bit the future
+ *    128 state outside the packed/rank domain, 256 owner-lane slab full
+ *    (sorted backend; the layer is repeated with more room).  On any error bit the future
  *    layer is invalid and the arena must not be resumed.
  *  - Engine entry points (mgbfs_engine_*) own their device memory, run on
  *    their own stream, and are not reentrant: one engine per host thread.

@@ -126,6 +126,34 @@ def run_sequential(instances, *, device=0, backend="permutation_bitmap"):
     return tables, time.perf_counter() - t0
 
 
+def run_sequential_reusing(instances, *, device=0, backend="permutation_bitmap"):
+    """Plain engine, one instance at a time, without per-instance device
+    allocation: one engine per distinct degree is created up front (outside
+    the timed region), and each instance swaps its generators into that
+    engine, which resets its bitmap and frontiers in place."""
+    from ._native import NativeEngine
+    engines = {}
+    for g in instances:
+        n = g["action"]["degree"]
+        if n not in engines:
+            engines[n] = NativeEngine(GraphDefinition(g), device=device, backend=backend)
+    tables = []
+    try:
+        t0 = time.perf_counter()
+        for g in instances:
+            e = engines[g["action"]["degree"]]
+            e.rebind(GraphDefinition(g))
+            status = e.run()
+            if status != 1:
+                raise RuntimeError(f"SEQUENTIAL_INSTANCE_STATUS {status}")
+            tables.append(e.layer_sizes())
+        seconds = time.perf_counter() - t0
+    finally:
+        for e in engines.values():
+            e.close()
+    return tables, seconds
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="batched vs one-at-a-time generator-set sweep")
