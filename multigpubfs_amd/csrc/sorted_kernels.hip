@@ -604,10 +604,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // Survivors are the occupied, unmarked slots; each entry carries the
     // merged back-edge mask of its duplicates.  Thread t owns slot pairs
-    // t + j * kUniqueThreads (conflict-free 16-byte reads).
+    // t + j * kUniqueThreads (conflict-free 16-byte reads).  The count
+    // atomic's return value is the survivor's rank within its fine bucket,
+    // kept (16 bits each, two per pair) for the write-out.
     uint32_t survive = 0, occupied = 0;
+    uint32_t rank[kPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kPairsPerThread; ++j) {
+      rank[j] = 0;
       const uint32_t i = threadIdx.x + j * kUniqueThreads;
       if (2 * i >= slots) continue;
       const ulonglong2 p = table2[i];
@@ -615,22 +619,25 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         ++occupied;
         if (!(p.x & a.mark)) {
           survive |= 1u << (2 * j);
-          atomicAdd(&u.fhist[fine_of(p.x, f)], 1u);
+          rank[j] = atomicAdd(&u.fhist[fine_of(p.x, f)], 1u);
         }
       }
       if (p.y != kEmpty) {
         ++occupied;
         if (!(p.y & a.mark)) {
           survive |= 2u << (2 * j);
-          atomicAdd(&u.fhist[fine_of(p.y, f)], 1u);
+          rank[j] |= atomicAdd(&u.fhist[fine_of(p.y, f)], 1u) << 16;
         }
       }
     }
-    if (occupied) atomicAdd(&u.unique, occupied);
+    // One LDS atomic per wave for the occupied-slot count.
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) occupied += __shfl_xor(occupied, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && occupied) atomicAdd(&u.unique, occupied);
     __syncthreads();
-    // One wave turns the fine counts into write cursors (exclusive scan by
-    // shuffles, <= 16 counters per lane) while the others wait at a single
-    // barrier.
+    // One wave turns the fine counts into the fine buckets' first rows
+    // (exclusive scan by shuffles, <= 16 counters per lane) while the
+    // others wait at a single barrier.
     if (threadIdx.x < kWave) {
       const uint32_t per = (fine + kWave - 1) / kWave, base = threadIdx.x * per;
       uint32_t sum = 0;
@@ -646,13 +653,14 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         }
     }
     __syncthreads();
-    // Survivors are re-read from the table and written back in place.
+    // Survivors are re-read from the table and written back in place at
+    // their fine bucket's first row plus their rank.
 #pragma unroll
     for (int j = 0; j < kPairsPerThread; ++j) {
       if (!((survive >> (2 * j)) & 3u)) continue;
       const ulonglong2 p = table2[threadIdx.x + j * kUniqueThreads];
-      if ((survive >> (2 * j)) & 1u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.x, f)], 1u)] = p.x;
-      if ((survive >> (2 * j)) & 2u) a.cand[s + atomicAdd(&u.fhist[fine_of(p.y, f)], 1u)] = p.y;
+      if ((survive >> (2 * j)) & 1u) a.cand[s + u.fhist[fine_of(p.x, f)] + (rank[j] & 0xffffu)] = p.x;
+      if ((survive >> (2 * j)) & 2u) a.cand[s + u.fhist[fine_of(p.y, f)] + (rank[j] >> 16)] = p.y;
     }
     if (threadIdx.x == 0) {
       unique_block += u.unique;
