@@ -16,7 +16,9 @@
  *    device word; bits: 1 table full, 2 visited capacity, 4 future capacity,
  *    8 bad row reference, 32 count above bound, 64 bucket above LDS capacity,
  *    128 state outside the packed/rank domain, 256 owner-lane slab full
- *    (sorted backend; the layer is repeated with more room).  On any error bit the future
+ *    (sorted backend; the layer is repeated with more room), 1024 segment
+ *    at its fill limit (segmented backend; the batch is replayed with a
+ *    larger segment).  On any error bit the future
  *    layer is invalid and the arena must not be resumed.
  *  - Engine entry points (mgbfs_engine_*) own their device memory, run on
  *    their own stream, and are not reentrant: one engine per host thread.
@@ -46,7 +48,8 @@ extern "C" {
 #define MGBFS_BACKEND_AUTO 0
 #define MGBFS_BACKEND_BITMAP 1   /* exact Lehmer-rank permutation bitmap          */
 #define MGBFS_BACKEND_SORTED 2   /* sort-unique + three rotating sorted banks     */
-#define MGBFS_BACKEND_SEGMENTED 3 /* sort-unique + segmented open-addressing hash */
+#define MGBFS_BACKEND_SEGMENTED 3 /* segmented open-addressing hash, any state width:
+                                     three rolling banks (inverse-closed) or full history */
 #define MGBFS_BACKEND_GENERIC 4  /* SoA int64/u8 states, all-visited exact hash   */
 
 /* Option flags. */
@@ -62,6 +65,12 @@ extern "C" {
                                              2 GiB (reference memory_preset "max_capacity") */
 #define MGBFS_FLAG_TEST_WIDE_ROUTING 32u  /* sorted backend: route with 64-bit slab addresses,
                                              the path taken by outputs of 2^32 entries or more */
+#define MGBFS_FLAG_TEST_TINY_SEGMENTS 128u /* segmented backend: every layer starts with a
+                                             1024-slot segment, so large layers spill into
+                                             growing segments */
+#define MGBFS_FLAG_TEST_SORTED_CAP 256u   /* sorted backend: layers of more than 2^18 children
+                                             count as beyond its limits, so the run is handed
+                                             to the segmented backend (as at F*G > 2^32) */
 
 /* Graph description: the schema-2 graph JSON of the reference
  * (multigpubfs/graph_definition.py:46-69) as plain arrays.  All pointers are

@@ -417,9 +417,12 @@ uint64_t state_space_bound(const Graph& g) {
 static uint32_t choose_backend(const Graph& g, uint32_t requested) {
   uint32_t backend = requested;
   if (backend == MGBFS_BACKEND_AUTO) {
-    // The sorted backend keeps at least 4 tag bits beside the 4n key bits.
+    // The sorted backend keeps at least 4 tag bits beside the 4n key bits;
+    // other inverse-closed graphs keep three rolling banks in segmented
+    // tables; other directed graphs keep their full history.
     if (g.packable() && g.inverse_closed && g.elements <= 15) backend = MGBFS_BACKEND_SORTED;
     else if (g.packable() && g.distinct_perm() && g.elements <= 15) backend = MGBFS_BACKEND_BITMAP;
+    else if (g.inverse_closed) backend = MGBFS_BACKEND_SEGMENTED;
     else backend = MGBFS_BACKEND_GENERIC;
   }
   switch (backend) {
@@ -431,11 +434,97 @@ static uint32_t choose_backend(const Graph& g, uint32_t requested) {
       if (!g.inverse_closed) throw std::runtime_error("SORTED_REQUIRES_INVERSE_CLOSED");
       return backend;
     case MGBFS_BACKEND_GENERIC:
+    case MGBFS_BACKEND_SEGMENTED:
       return backend;
     default:
       throw std::runtime_error("ENGINE_BACKEND_UNAVAILABLE");
   }
 }
+
+// The sorted backend, handing the run to the segmented backend when a
+// layer exceeds its limits (its last two layers become the segmented
+// backend's banks; the sorted engine and its memory are released).
+class HandoverEngine final : public Engine {
+ public:
+  HandoverEngine(const Graph& g, const mgbfs_engine_options& o) : g_(g), o_(o) {
+    inner_.reset(make_sorted_engine(g, o));
+    mirror();
+  }
+  int advance(uint64_t* next_count, uint32_t* fatal) override {
+    int s = inner_->advance(next_count, fatal);
+    if (s == kAdvanceHandover) {
+      PackedHandover h;
+      if (!inner_->handover(h)) throw std::runtime_error("ENGINE_HANDOVER");
+      const uint64_t cand = inner_->candidates, uniq = inner_->unique_candidates, launches_before = inner_->launches;
+      const double secs = inner_->kernel_seconds;
+      const uint64_t streamed = inner_->streamed_bytes;
+      std::unique_ptr<Engine> next(make_segmented_from_packed(g_, o_, state_space_bound(g_), h));
+      inner_ = std::move(next);
+      carried_ = {cand, uniq, launches_before, streamed, secs};
+      ++handovers_;
+      s = inner_->advance(next_count, fatal);
+    }
+    mirror();
+    return s;
+  }
+  void reset() override {
+    if (handovers_) {
+      inner_.reset();
+      inner_.reset(make_sorted_engine(g_, o_));
+      carried_ = {};
+    } else {
+      inner_->reset();
+    }
+    mirror();
+  }
+  void rebind(const Graph& g) override {
+    if (handovers_) throw std::runtime_error("ENGINE_REBIND_AFTER_HANDOVER");
+    inner_->rebind(g);
+    g_ = g;
+    mirror();
+  }
+  void layer_keys(std::vector<uint64_t>& out) override { inner_->layer_keys(out); }
+  void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) override {
+    inner_->sample(which, limit, rows, count);
+  }
+  const char* name() const override { return inner_->name(); }
+  uint64_t retry_count() const override { return inner_->retry_count(); }
+  void settle() override {
+    inner_->settle();
+    mirror();
+  }
+  void fill_info(mgbfs_engine_info& info) const override {
+    Engine::fill_info(info);
+    info.retries = inner_->retry_count();
+  }
+  void fill_phases(double* seconds, uint64_t* bytes, uint32_t n) const override { inner_->fill_phases(seconds, bytes, n); }
+
+ private:
+  struct Carried {
+    uint64_t candidates = 0, unique = 0, launches = 0, streamed = 0;
+    double seconds = 0.0;
+  };
+  // Counters of the sorted part of a handed-over run are added to the
+  // segmented engine's.
+  void mirror() {
+    layers = inner_->layers;
+    candidates = inner_->candidates + carried_.candidates;
+    unique_candidates = inner_->unique_candidates + carried_.unique;
+    device_bytes = inner_->device_bytes;
+    kernel_seconds = inner_->kernel_seconds + carried_.seconds;
+    launches = inner_->launches + carried_.launches;
+    streamed_bytes = inner_->streamed_bytes + carried_.streamed;
+    backend = inner_->backend;
+    history_layers = inner_->history_layers;
+    state_bytes = inner_->state_bytes;
+    terminal = inner_->terminal;
+  }
+  Graph g_;
+  mgbfs_engine_options o_;
+  std::unique_ptr<Engine> inner_;
+  Carried carried_;
+  uint32_t handovers_ = 0;
+};
 
 // Planned bytes per state of the widest layer on the sorted backend: three
 // banks grown by a quarter, plus per child (G-1 with back-edge masks, else
@@ -486,6 +575,23 @@ mgbfs_engine_plan_info plan_engine(const Graph& g, const mgbfs_engine_options& o
       p.device_bytes = fixed + per * cap;
       break;
     }
+    case MGBFS_BACKEND_SEGMENTED: {
+      // Rows of a layer (three banks) or of the whole history, with the
+      // segments at load factor 0.5 (16 bytes per row).
+      p.state_bytes = generic_state_bytes(g);
+      p.retained_banks = g.inverse_closed ? 3 : 0;
+      const uint64_t per = static_cast<uint64_t>(g.elements) * p.state_bytes + 16;
+      uint64_t cap = p.budget_bytes / (g.inverse_closed ? 3 * per : per);
+      if (p.state_bound) cap = std::min(cap, p.state_bound);
+      cap = std::min<uint64_t>(cap, 0xfffffffeull);
+      if (o.capacity) cap = std::min(cap, o.capacity);
+      if (!cap) throw std::runtime_error("SEGMENTED_VRAM_CAPACITY");
+      p.capacity = cap;
+      p.batch = std::max<uint64_t>(1, std::min<uint64_t>(cap, (1ull << 22) / G));
+      p.table_slots = 2 * cap;
+      p.device_bytes = (g.inverse_closed ? 3 : 1) * per * cap;
+      break;
+    }
     default: {
       p.state_bytes = generic_state_bytes(g);
       p.retained_banks = 0;
@@ -506,7 +612,9 @@ static Engine* create(const Graph& g, const mgbfs_engine_options& o) {
     case MGBFS_BACKEND_BITMAP:
       return make_bitmap_engine(g, o);
     case MGBFS_BACKEND_SORTED:
-      return make_sorted_engine(g, o);
+      return new HandoverEngine(g, o);
+    case MGBFS_BACKEND_SEGMENTED:
+      return make_segmented_engine(g, o, state_space_bound(g));
     default:
       return make_generic_engine(g, o, state_space_bound(g));
   }

@@ -78,6 +78,9 @@ class Engine {
   // Switches to another graph of the same shape, keeping every allocation,
   // and resets (backends that support it).
   virtual void rebind(const Graph&) { throw std::runtime_error("ENGINE_REBIND_UNSUPPORTED"); }
+  // Sorted backend: describes its last two layers after advance() returned
+  // kAdvanceHandover (its limits are exceeded) and frees its scratch.
+  virtual bool handover(struct PackedHandover&) { return false; }
   virtual void layer_keys(std::vector<uint64_t>& out) = 0;
   virtual void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) = 0;
   virtual const char* name() const = 0;
@@ -107,9 +110,28 @@ class Engine {
   hipStream_t stream_ = nullptr;
 };
 
+// Internal advance() status: the layer exceeds the sorted backend's limits
+// and the run continues on the segmented backend (never returned by the C ABI).
+constexpr int kAdvanceHandover = 3;
+
 Engine* make_bitmap_engine(const Graph& g, const mgbfs_engine_options& o);
 Engine* make_sorted_engine(const Graph& g, const mgbfs_engine_options& o);
 Engine* make_generic_engine(const Graph& g, const mgbfs_engine_options& o, uint64_t state_bound);
+Engine* make_segmented_engine(const Graph& g, const mgbfs_engine_options& o, uint64_t state_bound);
+// Layers d-1 and d of a sorted-backend run, handed to the segmented backend
+// when a layer exceeds the sorted backend's limits (device pointers to bank
+// values (mix(key) << tagbits) | tag; prev_count 0 when there is no d-1).
+struct PackedHandover {
+  const uint64_t* prev = nullptr;
+  uint64_t prev_count = 0;
+  const uint64_t* cur = nullptr;
+  uint64_t cur_count = 0;
+  KeyMix km{};
+  uint32_t tagbits = 0;
+  std::vector<uint64_t> layers;
+};
+Engine* make_segmented_from_packed(const Graph& g, const mgbfs_engine_options& o, uint64_t state_bound,
+                                   const PackedHandover& h);
 void set_last_error(const char* what);
 Graph intake_graph(const mgbfs_graph_desc* d);
 uint64_t vram_budget(const mgbfs_engine_options& o);

@@ -258,6 +258,7 @@ uint32_t backend_code(const std::string& name) {
   if (name == "auto") return MGBFS_BACKEND_AUTO;
   if (name == "permutation_bitmap" || name == "bitmap") return MGBFS_BACKEND_BITMAP;
   if (name == "sorted") return MGBFS_BACKEND_SORTED;
+  if (name == "segmented") return MGBFS_BACKEND_SEGMENTED;
   if (name == "generic" || name == "legacy") return MGBFS_BACKEND_GENERIC;
   throw CliError("CLI_GRAPH_BACKEND");
 }
@@ -296,15 +297,28 @@ bool path_exists(const std::string& path) {
 
 // Reference names of this library's visited-set algorithms and layouts.
 const char* history_algorithm(uint32_t backend) {
-  return backend == MGBFS_BACKEND_BITMAP ? "PERMUTATION_BITMAP" : backend == MGBFS_BACKEND_SORTED ? "SORTED_RUNS" : "HASH";
+  switch (backend) {
+    case MGBFS_BACKEND_BITMAP: return "PERMUTATION_BITMAP";
+    case MGBFS_BACKEND_SORTED: return "SORTED_RUNS";
+    case MGBFS_BACKEND_SEGMENTED: return "SEGMENTED_HASH";
+    default: return "HASH";
+  }
 }
 const char* table_layout(uint32_t backend) {
-  return backend == MGBFS_BACKEND_BITMAP   ? "LEHMER_RANK_BITMAP"
-         : backend == MGBFS_BACKEND_SORTED ? "SUB_BUCKET_LDS_TABLES"
-                                           : "OPEN_ADDRESSING_U64_SLOTS";
+  switch (backend) {
+    case MGBFS_BACKEND_BITMAP: return "LEHMER_RANK_BITMAP";
+    case MGBFS_BACKEND_SORTED: return "SUB_BUCKET_LDS_TABLES";
+    case MGBFS_BACKEND_SEGMENTED: return "BANK_SEGMENTS_U64_SLOTS";
+    default: return "OPEN_ADDRESSING_U64_SLOTS";
+  }
 }
 const char* backend_key(uint32_t backend) {
-  return backend == MGBFS_BACKEND_BITMAP ? "permutation_bitmap" : backend == MGBFS_BACKEND_SORTED ? "sorted" : "generic";
+  switch (backend) {
+    case MGBFS_BACKEND_BITMAP: return "permutation_bitmap";
+    case MGBFS_BACKEND_SORTED: return "sorted";
+    case MGBFS_BACKEND_SEGMENTED: return "segmented";
+    default: return "generic";
+  }
 }
 
 // Generator table bytes as the reference counts them (generic_distributed_run.rs:10).
@@ -1011,7 +1025,7 @@ int cmd_key_info() {
   out["schema"] = Json::integer(1);
   out["abi_version"] = Json::integer(mgbfs_hip_abi_version());
   Json b = Json::array();
-  for (const char* x : {"sorted", "permutation_bitmap", "generic"}) b.push(Json::str(x));
+  for (const char* x : {"sorted", "permutation_bitmap", "segmented", "generic"}) b.push(Json::str(x));
   out["backends"] = b;
   Json c = Json::array();
   for (const char* x : {"graph", "graph-info", "graph-plan", "graph-local-plan", "graph-rank", "graph-session",

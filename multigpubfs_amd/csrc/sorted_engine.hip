@@ -107,11 +107,9 @@ class SortedEngine final : public Engine {
     // sub-bucket count and the 32-bit candidate bound use that bound.
     const bool masked = layout_.prune && layers.size() > 1;
     const uint64_t Cmax = masked ? F * (G > 1 ? G - 1 : 1) : F * G;
-    if (Cmax > 0xfffffff0ull) {
-      *fatal = kErrFutureCap;
-      terminal = true;
-      return MGBFS_RESOURCE;
-    }
+    // Beyond 32-bit candidate counts the run continues on the segmented
+    // backend (a test flag lowers the limit to 2^18 children).
+    if (Cmax > ((flags_ & MGBFS_FLAG_TEST_SORTED_CAP) ? (1ull << 18) : 0xfffffff0ull)) return kAdvanceHandover;
     const int nxt = next_bank();
     uint64_t Cest = Cmax;
     // Masked layers skip every back-edge, so the previous layer's measured
@@ -231,6 +229,23 @@ class SortedEngine final : public Engine {
     gens_ = g.packed_generators();
     set_generators(g);
     reset();
+  }
+
+  bool handover(PackedHandover& h) override {
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    settle();
+    lane_.release();
+    sub_.release();
+    lane_cap_ = sub_cap_ = 0;
+    h.cur = bank_[cur_].as<uint64_t>();
+    h.cur_count = count_[cur_];
+    h.prev = prev_ >= 0 ? bank_[prev_].as<uint64_t>() : nullptr;
+    h.prev_count = prev_ >= 0 ? count_[prev_] : 0;
+    h.km = layout_.km;
+    h.tagbits = layout_.tagbits;
+    h.layers = layers;
+    update_bytes();
+    return true;
   }
 
   void layer_keys(std::vector<uint64_t>& out) override { read_bank(cur_, count_[cur_], out); }

@@ -20,12 +20,23 @@ from .graph_definition import GraphDefinition, from_cayleypy
 _MEMORY_BACKENDS = {
     "legacy": "auto",                  # best exact backend for the graph
     "auto": "auto",
-    "segmented": "sorted",             # sort-unique + three rotating banks
-    "segmented24": "sorted",
+    "segmented": "retained",           # three rotating banks: sorted when packable, else segmented hash
+    "segmented24": "retained",
     "sorted": "sorted",
+    "segmented_hash": "segmented",     # segmented open-addressing banks, any state width
     "permutation_bitmap": "permutation_bitmap",
     "generic": "generic",
 }
+
+
+def _retained_backend(graph) -> str:
+    """The reference's segmented (three-bank) memory backend on this build:
+    the sorted backend for inverse-closed permutations of degree <= 15 on
+    labels < 16, the segmented hash for every other graph."""
+    start = graph.start
+    packable = (graph.action["kind"] == "permutation" and len(start) <= 15
+                and all(0 <= v < 16 for v in start))
+    return "sorted" if packable and graph.inverse_closed() else "segmented"
 
 
 # Same file name as the reference's rank archives (specialized.py:164).
@@ -127,6 +138,8 @@ def run_graph(graph, output, *, device=None, devices=None, capacity=None, max_se
     if memory_preset == "max_capacity" and memory_backend not in ("segmented", "segmented24", "permutation_bitmap"):
         raise ValueError("MEMORY_PRESET_UNSUPPORTED_FOR_LEGACY")
     engine_backend = "generic" if backend == "generic" else _MEMORY_BACKENDS[memory_backend]
+    if engine_backend == "retained":
+        engine_backend = _retained_backend(graph)
     if engine_backend == "permutation_bitmap":
         if graph.action["kind"] != "permutation":
             raise ValueError("BITMAP_REQUIRES_PERMUTATION_ACTION")

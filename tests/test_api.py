@@ -353,7 +353,10 @@ def test_graph_plan_from_inventory_needs_no_device(tmp_path):
     sl2 = GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], p), ([1, 0, 1, 1], p), ([1, p - 1, 0, 1], p),
                                         ([1, 0, p - 1, 1], p)], [1, 0, 0, 1])
     directed = GraphDefinition.permutation([[1, 2, 3, 4, 5, 6, 0], [1, 0, 3, 2, 5, 4, 6]], list(range(7)))
-    for g, backend, history in ((lrx(13), "sorted", 3), (directed, "permutation_bitmap", 1), (sl2, "generic", 3)):
+    # Unipotent upper and lower shears without their inverses: directed.
+    shears = GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], p), ([1, 0, 1, 1], p)], [1, 0, 0, 1])
+    for g, backend, history in ((lrx(13), "sorted", 3), (directed, "permutation_bitmap", 1), (sl2, "segmented", 3),
+                                (shears, "generic", 1)):
         path = tmp_path / f"{backend}.json"
         path.write_text(g.to_json())
         proc = cli("graph-plan", path, inv, 4)
@@ -373,9 +376,9 @@ def test_graph_plan_from_inventory_needs_no_device(tmp_path):
     # plus the generator tables; capacity capped by the bound p^4.
     q = doc["plan"]
     assert q["capacity"] == p ** 4 and q["table_slots"] == 1 << 16 and q["state_bytes"] == 8
-    tables = 4 * 4 * 8 + 4 * 4
+    tables = 2 * 4 * 8 + 2 * 4
     assert q["device_bytes"] == 4 * p ** 4 * 8 * 2 + p ** 4 * 8 + (1 << 16) * 8 + 24 + tables
-    assert q["generator_bytes"] == 4 * (4 * 8 + 4)
+    assert q["generator_bytes"] == 2 * (4 * 8 + 4)
     bad = tmp_path / "bad.json"
     for entry, code in (({"device": 0, "free_bytes": 3 * gib, "total_bytes": 2 * gib}, "GRAPH_GLOBAL_MEMORY_RANGE"),
                         ({"device": 0, "free_bytes": gib, "total_bytes": 2 * gib}, "GENERIC_DISTRIBUTED_HEADROOM"),

@@ -151,7 +151,8 @@ def sl2(p):
 
 @pytest.mark.parametrize("name,graph,history,backend", [
     ("lrx8", lrx(8), 3, "sorted"), ("directed7", directed_graph(), 1, "permutation_bitmap"),
-    ("sl2_13", sl2(13), 3, "generic")])
+    ("sl2_13", sl2(13), 3, "segmented"),
+    ("shears_13", GraphDefinition.matrix(2, 2, [([1, 1, 0, 1], 13), ([1, 0, 1, 1], 13)], [1, 0, 0, 1]), 1, "generic")])
 def test_graph_info_admission_contract(tmp_path, name, graph, history, backend):
     # generic_distributed_run.rs:23-73: inventory per device, a rank plan per
     # device, the smallest as `plan`, owner cut points, reserve policy.
@@ -169,6 +170,8 @@ def test_graph_info_admission_contract(tmp_path, name, graph, history, backend):
     assert 0 < plan["batch"] <= plan["capacity"]
     assert plan["queue_capacity"] == plan["batch"] * len(graph.to_dict()["action"]["generators"])
     assert 0 < plan["device_bytes"] <= inv["free_bytes"]
+    if backend == "segmented":
+        assert plan["history_algorithm"] == "SEGMENTED_HASH" and plan["retained_banks"] == 3
     if backend == "generic":
         assert plan["table_slots"] >= 2 * plan["capacity"] and plan["history_algorithm"] == "HASH"
         # Capped by the state-space bound p^(rows*cols) of a single modulus
