@@ -46,6 +46,12 @@ struct BatchInstance {
   int64_t modulus;        // matrix: common modulus (> 0)
   uint8_t src[kBatchMaxGens][kMaxDegree];         // permutation tables
   int32_t mat[kBatchMaxGens][kBatchMaxEntries];   // matrix generators (row-major rows x rows)
+  // Parent-edge tags (bitmap instances of degree <= 15): a stored key
+  // carries 1 + the index of the inverse of the generator that made it in
+  // nibble 15, and expanding it skips that generator (its child is the
+  // parent, already visited).  tag_shift 64 = no tags.
+  uint8_t child_tag[kBatchMaxGens];
+  uint32_t tag_shift;
 };
 
 hipError_t batch_reset_flag(uint32_t* flag, hipStream_t stream);

@@ -216,12 +216,18 @@ class BatchEngine {
       d.table_mask = p.visited_bytes / 8 - 1;
       MGBFS_HIP(fill_u64(d.table, ~0ull, p.visited_bytes / 8, stream_));
     }
+    d.tag_shift = 64;
     if (g.kind == 0) {
       d.action = kActPerm;
       d.degree = g.elements;
       for (uint32_t k = 0; k < g.generators; ++k)
         for (uint32_t i = 0; i < g.elements; ++i) d.src[k][i] = static_cast<uint8_t>(g.perms[k * g.elements + i]);
       start_key = g.start_key();
+      if (p.visited == kVisBitmap && g.elements <= 15 && g.generators <= 14) {
+        const std::vector<int> inv = inverse_index(g.packed_generators());
+        d.tag_shift = 60;
+        for (uint32_t k = 0; k < g.generators; ++k) d.child_tag[k] = inv[k] >= 0 ? static_cast<uint8_t>(inv[k] + 1) : 0;
+      }
     } else {
       d.action = kActMatrix;
       d.rows = g.rows;

@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kBatchThreads = 256;
 constexpr int kBatchItems = 2;  // parents per thread per tile (tile = 512 parents)
+constexpr uint32_t kTileRun = 16;  // consecutive tiles one block takes in turn
 
 __device__ __forceinline__ int64_t pymod64(int64_t x, int64_t m) {
   int64_t r = x % m;
@@ -42,6 +43,41 @@ __device__ __forceinline__ uint64_t batch_apply(const BatchInstance& s, uint64_t
       child |= v << ((i * s.cols + j) * s.ebits);
     }
   return child;
+}
+
+// Bitmap claims of every child of one parent (generators other than the
+// one its tag names): ranks first, then every bitmap word load, then the
+// atomics for the clear bits, so a thread keeps all of a parent's probes in
+// flight at once.  Returns the claimed generators as a bit mask.
+__device__ __forceinline__ uint32_t batch_claim_bitmap(const BatchInstance& s, uint64_t key, uint32_t& err) {
+  const uint32_t G = s.generators;
+  const uint32_t skip = s.tag_shift < 64 ? static_cast<uint32_t>(key >> s.tag_shift) : 0u;
+  uint32_t word[kBatchMaxGens], bit[kBatchMaxGens], seen[kBatchMaxGens];
+  uint32_t live = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < kBatchMaxGens; ++g) {
+    word[g] = 0;
+    bit[g] = 0;
+    if (g >= G || g + 1 == skip) continue;
+    bool bad;
+    const uint64_t r = lehmer_rank_packed(apply_packed(key, s.src[g], s.degree), s.degree, &bad);
+    if (bad || r >= s.bitmap_bits) {
+      err |= kErrDomain;
+      continue;
+    }
+    word[g] = static_cast<uint32_t>(r >> 5);
+    bit[g] = 1u << (r & 31u);
+    live |= 1u << g;
+  }
+#pragma unroll
+  for (uint32_t g = 0; g < kBatchMaxGens; ++g) seen[g] = (live >> g) & 1u ? s.bitmap[word[g]] : 0u;
+  uint32_t claimed = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < kBatchMaxGens; ++g)
+    if (((live >> g) & 1u) && !(seen[g] & bit[g]) &&
+        !(__hip_atomic_fetch_or(s.bitmap + word[g], bit[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit[g]))
+      claimed |= 1u << g;
+  return claimed;
 }
 
 // Returns true when this thread claimed `key` (first visit).
@@ -114,15 +150,22 @@ __global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstan
   if (threadIdx.x == 0) staged = -1;
   __syncthreads();
   const uint32_t total = spre[slots];
-  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+  // Runs of kTileRun consecutive tiles are dealt to blocks round-robin:
+  // an instance's descriptor is staged once per run at most, and every
+  // block still samples all instances of the layer.
+  uint32_t i = 0;
+  for (uint32_t t0 = blockIdx.x * kTileRun; t0 < total; t0 += gridDim.x * kTileRun)
+  for (uint32_t t = t0; t < t0 + kTileRun && t < total; ++t) {
     // Instance owning tile t: the last i with spre[i] <= t.
-    uint32_t lo = 0, hi = slots - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) / 2;
-      if (spre[mid] <= t) lo = mid;
-      else hi = mid - 1;
+    if (!(spre[i] <= t && t < spre[i + 1])) {
+      uint32_t lo = 0, hi = slots - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (spre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      i = lo;
     }
-    const uint32_t i = lo;
     if (static_cast<int>(i) != staged) {
       __syncthreads();
       const uint32_t* src = reinterpret_cast<const uint32_t*>(inst + i);
@@ -142,8 +185,12 @@ __global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstan
       const uint64_t key = valid ? si.front[p] : 0;
       parents[it][threadIdx.x] = key;
       if (!valid) continue;
-      for (uint32_t g = 0; g < G; ++g)
-        if (batch_claim(si, batch_apply(si, key, g), err)) newmask |= 1u << (it * kBatchMaxGens + g);
+      if (si.visited == kVisBitmap) {
+        newmask |= batch_claim_bitmap(si, key, err) << (it * kBatchMaxGens);
+      } else {
+        for (uint32_t g = 0; g < G; ++g)
+          if (batch_claim(si, batch_apply(si, key, g), err)) newmask |= 1u << (it * kBatchMaxGens + g);
+      }
     }
     // Workgroup scan of new-child counts, one reservation per tile.
     const uint32_t mine = __popc(newmask);
@@ -173,7 +220,9 @@ __global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstan
         while (m) {
           const uint32_t b = __ffs(m) - 1;
           m &= m - 1;
-          si.next[pos++] = batch_apply(si, parents[b / kBatchMaxGens][threadIdx.x], b % kBatchMaxGens);
+          const uint32_t g = b % kBatchMaxGens;
+          const uint64_t tag = si.tag_shift < 64 ? static_cast<uint64_t>(si.child_tag[g]) << si.tag_shift : 0ull;
+          si.next[pos++] = batch_apply(si, parents[b / kBatchMaxGens][threadIdx.x], g) | tag;
         }
       }
     }
