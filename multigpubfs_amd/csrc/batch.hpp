@@ -45,6 +45,7 @@ struct BatchInstance {
   uint32_t ebits;         // matrix: bits per packed entry
   int64_t modulus;        // matrix: common modulus (> 0)
   uint8_t src[kBatchMaxGens][kMaxDegree];         // permutation tables
+  uint64_t src_word[kBatchMaxGens];               // the same tables, src[g][i] in nibble i
   int32_t mat[kBatchMaxGens][kBatchMaxEntries];   // matrix generators (row-major rows x rows)
   // Parent-edge tags (bitmap instances of degree <= 15): a stored key
   // carries 1 + the index of the inverse of the generator that made it in

@@ -221,7 +221,10 @@ class BatchEngine {
       d.action = kActPerm;
       d.degree = g.elements;
       for (uint32_t k = 0; k < g.generators; ++k)
-        for (uint32_t i = 0; i < g.elements; ++i) d.src[k][i] = static_cast<uint8_t>(g.perms[k * g.elements + i]);
+        for (uint32_t i = 0; i < g.elements; ++i) {
+          d.src[k][i] = static_cast<uint8_t>(g.perms[k * g.elements + i]);
+          d.src_word[k] |= static_cast<uint64_t>(g.perms[k * g.elements + i] & 15) << (4 * i);
+        }
       start_key = g.start_key();
       if (p.visited == kVisBitmap && g.elements <= 15 && g.generators <= 14) {
         const std::vector<int> inv = inverse_index(g.packed_generators());

@@ -28,8 +28,36 @@ __device__ __forceinline__ int64_t pymod64(int64_t x, int64_t m) {
   return r < 0 ? r + m : r;
 }
 
+// child nibble i = parent nibble src[i], the sources read from one packed word.
+__device__ __forceinline__ uint64_t apply_word(uint64_t key, uint64_t sw, uint32_t n) {
+  uint64_t child = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kMaxDegree; ++i)
+    if (i < n) child |= ((key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u) << (4u * i);
+  return child;
+}
+
+// Lehmer rank of the child apply_word(key, sw, n) without forming it; R is
+// uint32_t when n! < 2^32 (n <= 12).
+template <typename R>
+__device__ __forceinline__ R child_rank(uint64_t key, uint64_t sw, uint32_t n, bool& bad) {
+  uint32_t seen = 0;
+  R r = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kMaxDegree; ++i) {
+    if (i < n) {
+      const uint32_t v = static_cast<uint32_t>(key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u;
+      bad |= (v >= n) | ((seen >> v) & 1u);
+      const uint32_t d = v - static_cast<uint32_t>(__popc(seen & ((1u << v) - 1u)));
+      seen |= 1u << v;
+      r = r * static_cast<R>(n - i) + static_cast<R>(d);
+    }
+  }
+  return r;
+}
+
 __device__ __forceinline__ uint64_t batch_apply(const BatchInstance& s, uint64_t key, uint32_t g) {
-  if (s.action == kActPerm) return apply_packed(key, s.src[g], s.degree);
+  if (s.action == kActPerm) return apply_word(key, s.src_word[g], s.degree);
   const uint64_t emask = (1ull << s.ebits) - 1;
   uint64_t child = 0;
   for (uint32_t i = 0; i < s.rows; ++i)
@@ -46,37 +74,35 @@ __device__ __forceinline__ uint64_t batch_apply(const BatchInstance& s, uint64_t
 }
 
 // Bitmap claims of every child of one parent (generators other than the
-// one its tag names): ranks first, then every bitmap word load, then the
-// atomics for the clear bits, so a thread keeps all of a parent's probes in
-// flight at once.  Returns the claimed generators as a bit mask.
+// one its tag names), degree <= 12 so a rank is one 32-bit word: ranks
+// first, then every bitmap word load, then the atomics for the clear bits,
+// so a thread keeps all of a parent's probes in flight at once.  Returns
+// the claimed generators as a bit mask.
 __device__ __forceinline__ uint32_t batch_claim_bitmap(const BatchInstance& s, uint64_t key, uint32_t& err) {
-  const uint32_t G = s.generators;
+  constexpr uint32_t kNone = ~0u;  // above every rank (12! < 2^32 - 1)
+  const uint32_t G = s.generators, n = s.degree;
   const uint32_t skip = s.tag_shift < 64 ? static_cast<uint32_t>(key >> s.tag_shift) : 0u;
-  uint32_t word[kBatchMaxGens], bit[kBatchMaxGens], seen[kBatchMaxGens];
-  uint32_t live = 0;
+  uint32_t rank[kBatchMaxGens], seen[kBatchMaxGens];
 #pragma unroll
   for (uint32_t g = 0; g < kBatchMaxGens; ++g) {
-    word[g] = 0;
-    bit[g] = 0;
+    rank[g] = kNone;
     if (g >= G || g + 1 == skip) continue;
-    bool bad;
-    const uint64_t r = lehmer_rank_packed(apply_packed(key, s.src[g], s.degree), s.degree, &bad);
-    if (bad || r >= s.bitmap_bits) {
-      err |= kErrDomain;
-      continue;
-    }
-    word[g] = static_cast<uint32_t>(r >> 5);
-    bit[g] = 1u << (r & 31u);
-    live |= 1u << g;
+    bool bad = false;
+    const uint32_t r = child_rank<uint32_t>(key, s.src_word[g], n, bad);
+    if (bad || r >= s.bitmap_bits) err |= kErrDomain;
+    else rank[g] = r;
   }
 #pragma unroll
-  for (uint32_t g = 0; g < kBatchMaxGens; ++g) seen[g] = (live >> g) & 1u ? s.bitmap[word[g]] : 0u;
+  for (uint32_t g = 0; g < kBatchMaxGens; ++g) seen[g] = rank[g] != kNone ? s.bitmap[rank[g] >> 5] : 0u;
   uint32_t claimed = 0;
 #pragma unroll
-  for (uint32_t g = 0; g < kBatchMaxGens; ++g)
-    if (((live >> g) & 1u) && !(seen[g] & bit[g]) &&
-        !(__hip_atomic_fetch_or(s.bitmap + word[g], bit[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit[g]))
+  for (uint32_t g = 0; g < kBatchMaxGens; ++g) {
+    if (rank[g] == kNone) continue;
+    const uint32_t bit = 1u << (rank[g] & 31u);
+    if (!(seen[g] & bit) &&
+        !(__hip_atomic_fetch_or(s.bitmap + (rank[g] >> 5), bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
       claimed |= 1u << g;
+  }
   return claimed;
 }
 
@@ -137,7 +163,9 @@ __global__ __launch_bounds__(1024) void batch_tiles_kernel(const BatchInstance* 
   if (i == 0) prefix[slots] = total;
 }
 
-__global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstance* inst, uint32_t slots,
+// Eight waves per SIMD (64 VGPRs, no spills): the grid of 8 blocks per CU
+// is resident at once and the probes' latency is hidden across waves.
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8))) void batch_expand_kernel(BatchInstance* inst, uint32_t slots,
                                                                      const uint32_t* __restrict__ prefix,
                                                                      uint32_t tile) {
   __shared__ uint32_t spre[kBatchMaxSlots + 1];
@@ -150,12 +178,14 @@ __global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstan
   if (threadIdx.x == 0) staged = -1;
   __syncthreads();
   const uint32_t total = spre[slots];
-  // Runs of kTileRun consecutive tiles are dealt to blocks round-robin:
+  // Runs of up to kTileRun consecutive tiles are dealt to blocks round-robin:
   // an instance's descriptor is staged once per run at most, and every
   // block still samples all instances of the layer.
+  // Short layers take one tile per block so every block has work.
+  const uint32_t run = min(kTileRun, max(1u, total / gridDim.x));
   uint32_t i = 0;
-  for (uint32_t t0 = blockIdx.x * kTileRun; t0 < total; t0 += gridDim.x * kTileRun)
-  for (uint32_t t = t0; t < t0 + kTileRun && t < total; ++t) {
+  for (uint32_t t0 = blockIdx.x * run; t0 < total; t0 += gridDim.x * run)
+  for (uint32_t t = t0; t < t0 + run && t < total; ++t) {
     // Instance owning tile t: the last i with spre[i] <= t.
     if (!(spre[i] <= t && t < spre[i + 1])) {
       uint32_t lo = 0, hi = slots - 1;
@@ -185,7 +215,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_expand_kernel(BatchInstan
       const uint64_t key = valid ? si.front[p] : 0;
       parents[it][threadIdx.x] = key;
       if (!valid) continue;
-      if (si.visited == kVisBitmap) {
+      if (si.visited == kVisBitmap && si.degree <= 12) {
         newmask |= batch_claim_bitmap(si, key, err) << (it * kBatchMaxGens);
       } else {
         for (uint32_t g = 0; g < G; ++g)
