@@ -305,7 +305,9 @@ int mgbfs_batch_admit(void* batch, const mgbfs_graph_desc* graph, uint32_t visit
 int mgbfs_batch_step(void* batch, uint32_t layers, uint32_t* active);
 /* Steps until no instance is active. */
 int mgbfs_batch_run(void* batch);
-/* Slots that finished since the last poll. */
+/* Slots that finished since the last poll, at most `capacity` of them (later
+   polls report the rest); their layer tables are read back here, so
+   mgbfs_batch_result makes no device access for them. */
 int mgbfs_batch_poll(void* batch, int32_t* slots, uint32_t capacity, uint32_t* count);
 /* Layer table of a slot (depth 0 first); status 2 = done; error = sticky bits. */
 int mgbfs_batch_result(void* batch, int32_t slot, uint64_t* sizes, uint32_t capacity, uint32_t* count,

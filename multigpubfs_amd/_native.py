@@ -296,7 +296,7 @@ def graph_desc(graph):
     """mgbfs_graph_desc for a GraphDefinition (arrays kept alive on the object)."""
     from .graph_definition import GraphDefinition
     gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
-    data = gd.to_dict()
+    data = gd._data  # read only
     a = data["action"]
     desc = GraphDesc()
     desc.elements = len(data["start"])

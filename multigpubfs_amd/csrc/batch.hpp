@@ -55,11 +55,22 @@ struct BatchInstance {
   uint32_t tag_shift;
 };
 
+// Deferred descriptor upload: slot <- staged descriptor, then (seed != 0)
+// the start state is claimed and becomes layer 0.
+struct BatchInstall {
+  uint32_t slot, seed;
+  uint64_t start_key;
+};
+
 hipError_t batch_reset_flag(uint32_t* flag, hipStream_t stream);
 hipError_t batch_tiles(BatchInstance* inst, uint32_t slots, uint32_t tile, uint32_t* prefix, hipStream_t stream);
 hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t tile, uint32_t grid,
                         uint32_t threads, hipStream_t stream);
 hipError_t batch_finish(BatchInstance* inst, uint32_t slots, uint32_t* any_active, hipStream_t stream);
+hipError_t batch_gather_sizes(const BatchInstance* inst, const uint32_t* req, uint32_t n, uint64_t* out,
+                              hipStream_t stream);
+hipError_t batch_install(BatchInstance* inst, const BatchInstance* staged, const BatchInstall* req, uint32_t n,
+                         hipStream_t stream);
 hipError_t batch_seed(BatchInstance* inst, uint32_t slot, uint64_t start_key, hipStream_t stream);
 
 }  // namespace mgbfs

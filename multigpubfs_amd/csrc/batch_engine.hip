@@ -129,6 +129,14 @@ class BatchEngine {
     prefix_.allocate(4 * (slots_n_ + 1));
     flag_.allocate(64);
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_flag_), 64, hipHostMallocDefault));
+    MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_mirror_), sizeof(BatchInstance) * slots_n_,
+                            hipHostMallocDefault));
+    finished_.resize(slots_n_);
+    pending_.assign(slots_n_, kNothing);
+    pending_key_.assign(slots_n_, 0);
+    const size_t stage_bytes = (sizeof(BatchInstance) + sizeof(BatchInstall)) * slots_n_;
+    MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_bytes, hipHostMallocDefault));
+    stage_.allocate(stage_bytes);
     host_inst_.assign(slots_n_, BatchInstance{});
     std::memset(host_inst_.data(), 0, sizeof(BatchInstance) * slots_n_);
     MGBFS_HIP(hipMemcpy(inst_.as<void>(), host_inst_.data(), sizeof(BatchInstance) * slots_n_, hipMemcpyHostToDevice));
@@ -141,6 +149,10 @@ class BatchEngine {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (host_flag_) (void)hipHostFree(host_flag_);
+    if (host_mirror_) (void)hipHostFree(host_mirror_);
+    if (host_stage_) (void)hipHostFree(host_stage_);
+    if (host_req_) (void)hipHostFree(host_req_);
+    if (host_out_) (void)hipHostFree(host_out_);
     slots_.clear();
     arena_.trim();
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -244,11 +256,10 @@ class BatchEngine {
           d.mat[k][e] = static_cast<int32_t>(g.matrices[static_cast<size_t>(k) * g.rows * g.rows + e]);
       for (uint32_t e = 0; e < g.elements; ++e) start_key |= static_cast<uint64_t>(g.start[e]) << (e * eb);
     }
-    // Stream order makes the upload, fill and seed land before the next
-    // step's graph; host_inst_[slot] stays untouched until the next refresh,
-    // which synchronises, so no wait is needed here.
-    MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
-    MGBFS_HIP(batch_seed(inst_.as<BatchInstance>(), static_cast<uint32_t>(slot), start_key, stream_));
+    // The descriptor is installed and seeded by the next flush(), which the
+    // next step runs before its graph (stream order puts the fill first).
+    pending_[slot] = kInstallSeed;
+    pending_key_[slot] = start_key;
     sl.used = true;
     sl.reported = false;
     sl.planned = p.bytes;
@@ -265,6 +276,7 @@ class BatchEngine {
   uint32_t step(uint32_t layers) {
     MGBFS_HIP(hipSetDevice(device_));
     const uint32_t launches = std::max<uint32_t>(1, (layers + layers_per_graph_ - 1) / layers_per_graph_);
+    flush();
     for (uint32_t k = 0; k < launches; ++k) MGBFS_HIP(hipGraphLaunch(exec_, stream_));
     refresh();
     uint32_t active = 0;
@@ -278,22 +290,29 @@ class BatchEngine {
     }
   }
 
-  // Slots whose instance finished and has not been reported yet.
-  std::vector<int32_t> poll() {
+  // Slots whose instance finished and has not been reported yet (at most
+  // `limit`; the rest are reported by later polls).  Their layer tables are
+  // read back together, so result() needs no device access.
+  std::vector<int32_t> poll(uint32_t limit) {
     std::vector<int32_t> out;
-    for (uint32_t s = 0; s < slots_n_; ++s)
+    for (uint32_t s = 0; s < slots_n_ && out.size() < limit; ++s)
       if (slots_[s].used && !slots_[s].reported && (host_inst_[s].status & kInstDone)) {
         slots_[s].reported = true;
         out.push_back(static_cast<int32_t>(s));
       }
+    gather(out);
     return out;
   }
 
   void result(int32_t slot, std::vector<uint64_t>& sizes, uint32_t& status, uint32_t& error) {
     check_slot(slot);
     const BatchInstance& d = host_inst_[slot];
-    sizes.assign(d.depth + 1, 0);
-    MGBFS_HIP(hipMemcpy(sizes.data(), d.layer_sizes, sizes.size() * 8, hipMemcpyDeviceToHost));
+    if (finished_[slot].ready) {
+      sizes = finished_[slot].sizes;
+    } else {
+      sizes.assign(d.depth + 1, 0);
+      MGBFS_HIP(hipMemcpy(sizes.data(), d.layer_sizes, sizes.size() * 8, hipMemcpyDeviceToHost));
+    }
     status = d.status;
     error = d.error;
   }
@@ -320,12 +339,13 @@ class BatchEngine {
     Slot& sl = slots_[slot];
     BatchInstance& d = host_inst_[slot];
     std::memset(&d, 0, sizeof d);
-    // The cleared descriptor is uploaded before any later use of the block
-    // (same stream), so the range can be reused at once.
-    MGBFS_HIP(hipMemcpyAsync(inst_.as<BatchInstance>() + slot, &d, sizeof d, hipMemcpyHostToDevice, stream_));
+    // The cleared descriptor is installed by the next flush, before any
+    // graph reads descriptors again, so the range can be reused at once.
+    pending_[slot] = kInstallOnly;
     arena_.give(sl.block, budget_);
     used_ -= sl.planned;
     sl = Slot{};
+    finished_[slot] = Finished{};
   }
 
   uint64_t used() const { return used_; }
@@ -351,11 +371,78 @@ class BatchEngine {
     MGBFS_HIP(hipStreamEndCapture(stream_, &graph_));
     MGBFS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
   }
-  void refresh() {
-    MGBFS_HIP(hipMemcpyAsync(host_inst_.data(), inst_.as<void>(), sizeof(BatchInstance) * slots_n_,
-                             hipMemcpyDeviceToHost, stream_));
-    MGBFS_HIP(hipStreamSynchronize(stream_));
+  // Installs every descriptor admitted or released since the last flush
+  // (the latest content per slot) with one upload and one kernel.  The
+  // pinned staging area is reused only after refresh() has waited.
+  void flush() {
+    uint32_t n = 0;
+    BatchInstall* req = reinterpret_cast<BatchInstall*>(host_stage_ + slots_n_);
+    for (uint32_t s = 0; s < slots_n_; ++s) {
+      if (pending_[s] == kNothing) continue;
+      host_stage_[n] = host_inst_[s];
+      req[n] = BatchInstall{s, pending_[s] == kInstallSeed ? 1u : 0u, pending_key_[s]};
+      pending_[s] = kNothing;
+      ++n;
+    }
+    if (n == 0) return;
+    const size_t bytes = sizeof(BatchInstance) * slots_n_ + sizeof(BatchInstall) * n;
+    MGBFS_HIP(hipMemcpyAsync(stage_.as<void>(), host_stage_, bytes, hipMemcpyHostToDevice, stream_));
+    const BatchInstance* staged = stage_.as<BatchInstance>();
+    MGBFS_HIP(batch_install(inst_.as<BatchInstance>(), staged,
+                            reinterpret_cast<const BatchInstall*>(staged + slots_n_), n, stream_));
   }
+  void refresh() {
+    MGBFS_HIP(hipMemcpyAsync(host_mirror_, inst_.as<void>(), sizeof(BatchInstance) * slots_n_, hipMemcpyDeviceToHost,
+                             stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(host_inst_.data(), host_mirror_, sizeof(BatchInstance) * slots_n_);
+  }
+  // Layer tables of the given finished slots: one request upload, one gather
+  // kernel and one download, then a single wait.
+  void gather(const std::vector<int32_t>& done) {
+    if (done.empty()) return;
+    uint64_t total = 0;
+    for (int32_t slot : done) total += host_inst_[slot].depth + 1ull;
+    if (total > 0xffffffffull) throw std::runtime_error("BATCH_GATHER_SIZE");
+    const size_t req_bytes = 12 * done.size(), out_bytes = 8 * total;
+    if (req_bytes > req_cap_) {
+      if (host_req_) MGBFS_HIP(hipHostFree(host_req_));
+      host_req_ = nullptr;
+      MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_req_), req_bytes, hipHostMallocDefault));
+      gather_req_.allocate(req_bytes);
+      req_cap_ = req_bytes;
+    }
+    if (out_bytes > out_cap_) {
+      if (host_out_) MGBFS_HIP(hipHostFree(host_out_));
+      host_out_ = nullptr;
+      MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_out_), out_bytes, hipHostMallocDefault));
+      gather_out_.allocate(out_bytes);
+      out_cap_ = out_bytes;
+    }
+    uint32_t off = 0;
+    for (size_t q = 0; q < done.size(); ++q) {
+      const uint32_t count = host_inst_[done[q]].depth + 1;
+      host_req_[3 * q] = static_cast<uint32_t>(done[q]);
+      host_req_[3 * q + 1] = off;
+      host_req_[3 * q + 2] = count;
+      off += count;
+    }
+    MGBFS_HIP(hipMemcpyAsync(gather_req_.as<void>(), host_req_, req_bytes, hipMemcpyHostToDevice, stream_));
+    MGBFS_HIP(batch_gather_sizes(inst_.as<BatchInstance>(), gather_req_.as<uint32_t>(),
+                                 static_cast<uint32_t>(done.size()), gather_out_.as<uint64_t>(), stream_));
+    MGBFS_HIP(hipMemcpyAsync(host_out_, gather_out_.as<void>(), out_bytes, hipMemcpyDeviceToHost, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    for (size_t q = 0; q < done.size(); ++q) {
+      Finished& f = finished_[done[q]];
+      f.sizes.assign(host_out_ + host_req_[3 * q + 1], host_out_ + host_req_[3 * q + 1] + host_req_[3 * q + 2]);
+      f.ready = true;
+    }
+  }
+
+  struct Finished {
+    std::vector<uint64_t> sizes;
+    bool ready = false;
+  };
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
@@ -363,6 +450,17 @@ class BatchEngine {
   uint64_t budget_ = 0, used_ = 0;
   DeviceBuffer inst_, prefix_, flag_;
   uint32_t* host_flag_ = nullptr;
+  BatchInstance* host_mirror_ = nullptr;  // pinned target of refresh()
+  enum Pending : uint8_t { kNothing = 0, kInstallSeed = 1, kInstallOnly = 2 };
+  std::vector<uint8_t> pending_;          // per slot: descriptor awaiting flush()
+  std::vector<uint64_t> pending_key_;
+  BatchInstance* host_stage_ = nullptr;   // pinned: slots_n_ descriptors, then the requests
+  DeviceBuffer stage_;
+  uint32_t* host_req_ = nullptr;          // pinned gather requests
+  uint64_t* host_out_ = nullptr;          // pinned gathered layer tables
+  size_t req_cap_ = 0, out_cap_ = 0;
+  DeviceBuffer gather_req_, gather_out_;
+  std::vector<Finished> finished_;
   std::vector<BatchInstance> host_inst_;
   std::vector<Slot> slots_;
   std::map<int32_t, uint32_t> kinds_, rows_, cols_;
@@ -445,11 +543,10 @@ extern "C" int mgbfs_batch_run(void* batch) {
 extern "C" int mgbfs_batch_poll(void* batch, int32_t* slots, uint32_t capacity, uint32_t* count) {
   if (!batch || !count) return hipErrorInvalidValue;
   MGBFS_BATCH_GUARD({
-    const std::vector<int32_t> done = static_cast<BatchEngine*>(batch)->poll();
+    if (!slots && capacity) return static_cast<int>(hipErrorInvalidValue);
+    const std::vector<int32_t> done = static_cast<BatchEngine*>(batch)->poll(capacity);
     *count = static_cast<uint32_t>(done.size());
-    if (done.size() > capacity) return static_cast<int>(hipErrorInvalidValue);
-    if (slots)
-      for (size_t k = 0; k < done.size(); ++k) slots[k] = done[k];
+    for (size_t k = 0; k < done.size(); ++k) slots[k] = done[k];
     return 0;
   })
 }

@@ -12,6 +12,7 @@
 //                  appended with one atomic per workgroup.
 //   batch_finish : one thread per instance records the layer size, swaps the
 //                  frontier buffers and retires finished instances.
+#include <algorithm>
 #include "batch.hpp"
 #include "kernels.hpp"
 
@@ -290,9 +291,17 @@ __global__ void batch_finish_kernel(BatchInstance* inst, uint32_t slots, uint32_
   atomicOr(any_active, 1u);
 }
 
-__global__ void batch_seed_kernel(BatchInstance* inst, uint32_t slot, uint64_t start_key) {
-  if (threadIdx.x != 0) return;
-  BatchInstance& s = inst[slot];
+// Layer tables of finished instances into one staging buffer: request q is
+// (slot, offset, count) and copies layer_sizes[0, count) to out[offset...].
+__global__ void batch_gather_sizes_kernel(const BatchInstance* inst, const uint32_t* req, uint32_t n, uint64_t* out) {
+  for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+    const uint64_t* sizes = inst[req[3 * q]].layer_sizes;
+    const uint32_t off = req[3 * q + 1], count = req[3 * q + 2];
+    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) out[off + k] = sizes[k];
+  }
+}
+
+__device__ void seed_instance(BatchInstance& s, uint64_t start_key) {
   uint32_t err = 0;
   if (!batch_claim(s, start_key, err)) err |= kErrDomain;
   s.front[0] = start_key;
@@ -302,6 +311,23 @@ __global__ void batch_seed_kernel(BatchInstance* inst, uint32_t slot, uint64_t s
   s.layer_sizes[0] = 1;
   s.error = err;
   s.status = err ? kInstDone : kInstActive;
+}
+
+__global__ void batch_seed_kernel(BatchInstance* inst, uint32_t slot, uint64_t start_key) {
+  if (threadIdx.x == 0) seed_instance(inst[slot], start_key);
+}
+
+// Block q writes staged descriptor q into its slot and, when asked, seeds it.
+__global__ void batch_install_kernel(BatchInstance* inst, const BatchInstance* staged, const BatchInstall* req,
+                                     uint32_t n) {
+  for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(staged + q);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(inst + req[q].slot);
+    for (uint32_t w = threadIdx.x; w < sizeof(BatchInstance) / 4; w += blockDim.x) dst[w] = src[w];
+    __syncthreads();
+    if (threadIdx.x == 0 && req[q].seed) seed_instance(inst[req[q].slot], req[q].start_key);
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -327,6 +353,22 @@ hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* pre
 
 hipError_t batch_finish(BatchInstance* inst, uint32_t slots, uint32_t* any_active, hipStream_t stream) {
   hipLaunchKernelGGL(batch_finish_kernel, dim3((slots + 255) / 256), dim3(256), 0, stream, inst, slots, any_active);
+  return hipGetLastError();
+}
+
+hipError_t batch_gather_sizes(const BatchInstance* inst, const uint32_t* req, uint32_t n, uint64_t* out,
+                              hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_gather_sizes_kernel, dim3(std::min<uint32_t>(n, 1024)), dim3(64), 0, stream, inst, req, n,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t batch_install(BatchInstance* inst, const BatchInstance* staged, const BatchInstall* req, uint32_t n,
+                         hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_install_kernel, dim3(std::min<uint32_t>(n, 1024)), dim3(64), 0, stream, inst, staged, req,
+                     n);
   return hipGetLastError();
 }
 

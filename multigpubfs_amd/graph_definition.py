@@ -9,7 +9,6 @@ product: it lives in oracle/ as test infrastructure.
 """
 from __future__ import annotations
 
-import copy
 import hashlib
 import json
 
@@ -35,12 +34,23 @@ def _int64_list(values):
     return [_int64(v) for v in values]
 
 
+def _plain_copy(value):
+    """Copy of a JSON-shaped value (dicts, lists, tuples of scalars); other
+    objects are shared, as they are immutable scalars in a valid graph."""
+    if isinstance(value, dict):
+        return {k: _plain_copy(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        out = [v if isinstance(v, (int, str, float)) or v is None else _plain_copy(v) for v in value]
+        return out if isinstance(value, list) else tuple(out)
+    return value
+
+
 class GraphDefinition:
     """Schema-2 graph: ``{schema, name, generator_names, action, start,
     expected_max_unique_states}`` with a permutation or matrix action."""
 
     def __init__(self, value: dict):
-        self._data = copy.deepcopy(value)
+        self._data = _plain_copy(value)
         self.validate()
 
     # -- constructors ----------------------------------------------------
@@ -66,11 +76,11 @@ class GraphDefinition:
     # -- accessors -------------------------------------------------------
     @property
     def start(self):
-        return copy.deepcopy(self._data["start"])
+        return _plain_copy(self._data["start"])
 
     @property
     def action(self):
-        return copy.deepcopy(self._data["action"])
+        return _plain_copy(self._data["action"])
 
     @property
     def name(self):
@@ -89,7 +99,7 @@ class GraphDefinition:
         return len(self._data["action"]["generators"])
 
     def to_dict(self):
-        return copy.deepcopy(self._data)
+        return _plain_copy(self._data)
 
     def to_json(self):
         return json.dumps(self._data, sort_keys=True, separators=(",", ":"), ensure_ascii=True)
