@@ -25,7 +25,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # Calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE per kernel for one full BFS,
 # committed beside the text summaries (profiles/*_s13_lrx.md).
-TRAFFIC_FILES = {"sorted": ROOT / "profiles" / "sorted_engine_s13_lrx.json"}
+TRAFFIC_FILES = {"sorted": ROOT / "profiles" / "sorted_engine_s13_lrx.json",
+                 "bitmap": ROOT / "profiles" / "bitmap_expand_kernel_s13_lrx.json"}
 REFERENCE_EDGES_PER_S = 18681062400 / 11.716894  # 8x H200 median, docs/validation/lrx13-eight-h200.md:26
 
 
