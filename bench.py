@@ -64,7 +64,7 @@ def sweep_section() -> dict:
     from multigpubfs_amd.sweep import run_batched, run_sequential, run_sequential_reusing
     golden = json.loads((ROOT / "tests" / "golden" / "sweep256.json").read_text())["instances"]
     inst = [x["graph"] for x in golden]
-    from multigpubfs_amd.autotune import choose_profile
+    from multigpubfs_amd.autotune import DEFAULT as DEFAULT_PROFILE, choose_profile
     t0 = time.perf_counter()
     profile = choose_profile(inst, use_cache=False)
     tune_s = time.perf_counter() - t0
@@ -81,7 +81,8 @@ def sweep_section() -> dict:
                                            "reused by every instance of that degree"},
             "identical_tables": batched == seq == reuse, "golden_exact": batched == want,
             "states": sum(sum(w) for w in want), "autotune_seconds": tune_s,
-            "profile": {k: profile[k] for k in ("grid_per_cu", "layers_per_graph", "visited")}}
+            "modes": info["modes"], "restarts": info["restarts"],
+            "profile": {k: profile[k] for k in DEFAULT_PROFILE}}
 
 
 def serving_checks(jobs, results) -> dict:

@@ -291,6 +291,17 @@ typedef struct mgbfs_batch_options {
   uint64_t vram_budget;      /* bytes for admitted jobs (0 = free - 4 GiB)      */
   uint32_t layers_per_graph; /* layers captured per hipGraph launch (0 = 4)     */
   uint32_t grid;             /* persistent expand grid (0 = 8 x CUs)            */
+  uint32_t threads;          /* expand workgroup: 128, 256 or 512 (0 = 256)     */
+  uint32_t hash_load_pct;    /* hash tables filled to at most this % (0 = 50)   */
+  uint32_t history;          /* 0: three rolling tables for inverse-closed hash
+                                instances, full history otherwise; 1: full
+                                history for every instance                     */
+  uint32_t layer_pct;        /* rolling instances: layer capacity as % of the
+                                state bound (0 = 30); a layer that outgrows it
+                                restarts the instance with twice the room      */
+  uint32_t bitmap_bias_permille; /* visited 0: bitmap when its bytes are at most
+                                this many thousandths of the hash tables'
+                                (0 = 1000)                                     */
 } mgbfs_batch_options;
 
 int mgbfs_batch_create(const mgbfs_batch_options* options, void** batch, char* error_text, size_t error_capacity);
@@ -316,6 +327,10 @@ int mgbfs_batch_result(void* batch, int32_t slot, uint64_t* sizes, uint32_t capa
 int mgbfs_batch_sample(void* batch, int32_t slot, uint64_t limit, int64_t* rows, uint64_t* count);
 int mgbfs_batch_release(void* batch, int32_t slot);
 int mgbfs_batch_usage(void* batch, uint64_t* used, uint64_t* budget, uint32_t* slots);
+/* An admitted slot's visited set (1 bitmap, 2 hash), retained layers (3 =
+   rolling tables, 0 = full history), layer capacity and restarts. */
+int mgbfs_batch_instance_info(void* batch, int32_t slot, uint32_t* visited, uint32_t* history_layers,
+                              uint64_t* layer_capacity, uint32_t* restarts);
 void mgbfs_batch_destroy(void* batch);
 
 /* --------------------------------------------- generic SoA kernel ABI */

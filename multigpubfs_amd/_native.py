@@ -230,7 +230,9 @@ def device_info(device: int = 0) -> dict:
 
 class BatchOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_slots", C.c_uint32), ("vram_budget", C.c_uint64),
-                ("layers_per_graph", C.c_uint32), ("grid", C.c_uint32)]
+                ("layers_per_graph", C.c_uint32), ("grid", C.c_uint32), ("threads", C.c_uint32),
+                ("hash_load_pct", C.c_uint32), ("history", C.c_uint32), ("layer_pct", C.c_uint32),
+                ("bitmap_bias_permille", C.c_uint32)]
 
 
 class SortedPlanInfo(C.Structure):
@@ -284,9 +286,12 @@ def _bind_batch(lib):
     lib.mgbfs_batch_sample.argtypes = [p, i32, u64, C.POINTER(C.c_int64), C.POINTER(u64)]
     lib.mgbfs_batch_release.argtypes = [p, i32]
     lib.mgbfs_batch_usage.argtypes = [p, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]
+    lib.mgbfs_batch_instance_info.argtypes = [p, i32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64),
+                                              C.POINTER(u32)]
     lib.mgbfs_batch_destroy.argtypes = [p]
     lib.mgbfs_batch_destroy.restype = None
-    for name in ("create", "plan", "admit", "step", "run", "poll", "result", "sample", "release", "usage"):
+    for name in ("create", "plan", "admit", "step", "run", "poll", "result", "sample", "release", "usage",
+                 "instance_info"):
         getattr(lib, "mgbfs_batch_" + name).restype = C.c_int
     lib._batch_bound = True
     return lib
@@ -322,9 +327,16 @@ def graph_desc(graph):
 class BatchEngine:
     """Many BFS instances on one GPU, advanced together (mgbfs_batch_*)."""
 
-    def __init__(self, *, device=0, max_slots=256, vram_budget=0, layers_per_graph=0, grid=0):
+    def __init__(self, *, device=0, max_slots=256, vram_budget=0, layers_per_graph=0, grid=0, threads=0,
+                 hash_load_pct=0, history="auto", layer_pct=0, bitmap_bias_permille=0):
+        """history: "auto" (three rolling tables for inverse-closed hash
+        instances, full history otherwise) or "full"; zeros take the
+        engine's defaults (mgbfs_batch_options in include/mgbfs_hip.h)."""
         self._lib = _bind_batch(load())
-        opts = BatchOptions(device, max_slots, vram_budget, layers_per_graph, grid)
+        if history not in ("auto", "full"):
+            raise ValueError("BATCH_OPTION_HISTORY")
+        opts = BatchOptions(device, max_slots, vram_budget, layers_per_graph, grid, threads, hash_load_pct,
+                            1 if history == "full" else 0, layer_pct, bitmap_bias_permille)
         h = C.c_void_p()
         err = C.create_string_buffer(256)
         if self._lib.mgbfs_batch_create(C.byref(opts), C.byref(h), err, 256):
@@ -398,6 +410,16 @@ class BatchEngine:
 
     def release(self, slot):
         self._check(self._lib.mgbfs_batch_release(self._h, slot), "BATCH_RELEASE")
+
+    def instance_info(self, slot):
+        """{visited: bitmap|hash, history_layers: 3 (rolling) or 0 (full),
+        layer_capacity, restarts} of an admitted slot."""
+        v, h, r = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        cap = C.c_uint64()
+        self._check(self._lib.mgbfs_batch_instance_info(self._h, slot, C.byref(v), C.byref(h), C.byref(cap),
+                                                        C.byref(r)), "BATCH_INSTANCE_INFO")
+        return {"visited": "bitmap" if v.value == 1 else "hash", "history_layers": h.value,
+                "layer_capacity": cap.value, "restarts": r.value}
 
     def usage(self):
         used, budget, slots = C.c_uint64(), C.c_uint64(), C.c_uint32()

@@ -36,7 +36,15 @@ struct BatchInstance {
   uint32_t* bitmap;       // kVisBitmap: n! bits (Lehmer rank)
   uint64_t bitmap_bits;
   uint64_t* table;        // kVisHash: open addressing, EMPTY = ~0, key in slot
-  uint64_t table_mask;
+  uint64_t table_slots;   // slots per table (home slot = high half of mix64(key) * slots)
+  // Three rolling tables (inverse-closed hash instances): layer d's states
+  // are claimed in table d % 3 at table + (d % 3) * table_stride; a child of
+  // layer d is new iff it is in neither table (d - 1) % 3 nor d % 3, which
+  // is exact because an undirected edge joins states at most one layer
+  // apart.  Table (d + 1) % 3 is cleared before layer d is expanded.
+  uint32_t rolling;
+  uint32_t pad1;
+  uint64_t table_stride;
   // Action.
   uint32_t action;        // kActPerm / kActMatrix
   uint32_t degree;        // permutation degree n
@@ -63,9 +71,15 @@ struct BatchInstall {
 };
 
 hipError_t batch_reset_flag(uint32_t* flag, hipStream_t stream);
+// prefix: 2 * (slots + 1) words (expansion tiles, then clear work items).
 hipError_t batch_tiles(BatchInstance* inst, uint32_t slots, uint32_t tile, uint32_t* prefix, hipStream_t stream);
+// threads: 128, 256 or 512 per workgroup; tile = 2 * threads parents.
 hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t tile, uint32_t grid,
                         uint32_t threads, hipStream_t stream);
+// Clears the table the next layer of every active rolling instance claims in
+// (work items from the second half of batch_tiles' prefix).
+hipError_t batch_clear(const BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t grid,
+                       hipStream_t stream);
 hipError_t batch_finish(BatchInstance* inst, uint32_t slots, uint32_t* any_active, hipStream_t stream);
 hipError_t batch_gather_sizes(const BatchInstance* inst, const uint32_t* req, uint32_t n, uint64_t* out,
                               hipStream_t stream);

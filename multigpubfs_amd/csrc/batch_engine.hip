@@ -1,6 +1,6 @@
 // Host side of the batched multi-instance engine: slot admission with VRAM
 // planning, a pooled device allocator, and a hipGraph-captured layer loop
-// (reset flag, then K x [tiles, expand, finish]) replayed until no instance
+// (reset flag, then K x [tiles, clear, expand, finish]) replayed until no instance
 // is active.  Instances join and leave at layer boundaries, so the same
 // object serves both the fixed sweep and continuous-batching serving.
 #include <algorithm>
@@ -19,7 +19,6 @@ namespace {
 
 constexpr uint32_t kLayersPerGraph = 4;
 constexpr uint32_t kDefaultMaxLayers = 4096;
-constexpr uint32_t kTile = 512;
 
 uint64_t next_pow2(uint64_t x) {
   uint64_t p = 1;
@@ -105,7 +104,8 @@ struct Slot {
 
 struct BatchPlan {
   uint32_t visited = kVisBitmap;
-  uint64_t bound = 0, cap = 0, visited_bytes = 0, bytes = 0;
+  bool rolling = false;  // three rolling hash tables, layers of at most `cap` states
+  uint64_t bound = 0, cap = 0, table_slots = 0, visited_bytes = 0, bytes = 0;
   uint32_t max_layers = kDefaultMaxLayers;
 };
 
@@ -124,14 +124,25 @@ class BatchEngine {
     hipDeviceProp_t prop;
     MGBFS_HIP(hipGetDeviceProperties(&prop, device_));
     grid_ = o.grid ? o.grid : static_cast<uint32_t>(prop.multiProcessorCount) * 8;
+    clear_grid_ = static_cast<uint32_t>(prop.multiProcessorCount) * 4;
     layers_per_graph_ = o.layers_per_graph ? o.layers_per_graph : kLayersPerGraph;
+    threads_ = o.threads ? o.threads : 256;
+    if (threads_ != 128 && threads_ != 256 && threads_ != 512) throw std::runtime_error("BATCH_OPTION_THREADS");
+    hash_load_pct_ = o.hash_load_pct ? o.hash_load_pct : 50;
+    if (hash_load_pct_ < 10 || hash_load_pct_ > 90) throw std::runtime_error("BATCH_OPTION_HASH_LOAD");
+    if (o.history > 1) throw std::runtime_error("BATCH_OPTION_HISTORY");
+    full_history_ = o.history == 1;
+    layer_pct_ = o.layer_pct ? o.layer_pct : 30;
+    if (layer_pct_ > 100) throw std::runtime_error("BATCH_OPTION_LAYER_PCT");
+    bitmap_bias_permille_ = o.bitmap_bias_permille ? o.bitmap_bias_permille : 1000;
     inst_.allocate(sizeof(BatchInstance) * slots_n_);
-    prefix_.allocate(4 * (slots_n_ + 1));
+    prefix_.allocate(8 * (slots_n_ + 1));
     flag_.allocate(64);
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_flag_), 64, hipHostMallocDefault));
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_mirror_), sizeof(BatchInstance) * slots_n_,
                             hipHostMallocDefault));
     finished_.resize(slots_n_);
+    jobs_.resize(slots_n_);
     pending_.assign(slots_n_, kNothing);
     pending_key_.assign(slots_n_, 0);
     const size_t stage_bytes = (sizeof(BatchInstance) + sizeof(BatchInstall)) * slots_n_;
@@ -159,7 +170,14 @@ class BatchEngine {
   }
 
   // Planned footprint (the admission unit) and the visited-set choice.
-  BatchPlan plan(const Graph& g, uint32_t visited, uint32_t max_layers) const {
+  // Hash instances of inverse-closed graphs keep three rolling tables
+  // (history_layers 3) sized, like their frontiers, for layers of up to
+  // layer_pct % of the state bound, times `scale` after restarts; others
+  // keep one table for the whole orbit.  Tables hold at most hash_load_pct %
+  // of their slots.  The automatic visited choice takes the Lehmer bitmap
+  // when its bytes are at most bitmap_bias_permille / 1000 of the hash
+  // tables'.
+  BatchPlan plan(const Graph& g, uint32_t visited, uint32_t max_layers, uint32_t scale = 1) const {
     BatchPlan p;
     p.max_layers = max_layers ? max_layers : kDefaultMaxLayers;
     if (g.kind == 0) {
@@ -183,12 +201,31 @@ class BatchEngine {
     if (!p.bound || p.bound > (1ull << 33)) throw std::runtime_error("BATCH_STATE_BOUND");
     const bool bitmap_ok = g.kind == 0 && g.distinct_perm() && factorial_u64(g.elements) <= (1ull << 34);
     const uint64_t bitmap_bytes = bitmap_ok ? (factorial_u64(g.elements) + 31) / 32 * 4 : ~0ull;
-    const uint64_t hash_bytes = next_pow2(2 * p.bound) * 8;
-    if (visited == 0) p.visited = bitmap_ok && bitmap_bytes <= hash_bytes ? kVisBitmap : kVisHash;
-    else p.visited = visited == 1 ? kVisBitmap : kVisHash;
+    const bool rolling = !full_history_ && g.inverse_closed;
+    uint64_t cap = p.bound;
+    if (rolling) {
+      const uint64_t want = (p.bound * layer_pct_ + 99) / 100 * scale;
+      cap = std::min(p.bound, std::max<uint64_t>(1024, want));
+    }
+    const uint64_t slots = std::max<uint64_t>(64, (cap * 100 + hash_load_pct_ - 1) / hash_load_pct_);
+    const uint64_t hash_bytes = (rolling ? 3 : 1) * slots * 8;
+    if (visited == 0)
+      p.visited = bitmap_ok && static_cast<double>(bitmap_bytes) * 1000.0 <=
+                                   static_cast<double>(hash_bytes) * bitmap_bias_permille_
+                      ? kVisBitmap
+                      : kVisHash;
+    else
+      p.visited = visited == 1 ? kVisBitmap : kVisHash;
     if (p.visited == kVisBitmap && !bitmap_ok) throw std::runtime_error("BATCH_BITMAP_UNSUPPORTED");
-    p.visited_bytes = p.visited == kVisBitmap ? bitmap_bytes : hash_bytes;
-    p.cap = p.bound;
+    if (p.visited == kVisBitmap) {
+      p.visited_bytes = bitmap_bytes;
+      p.cap = p.bound;
+    } else {
+      p.rolling = rolling;
+      p.table_slots = slots;
+      p.visited_bytes = hash_bytes;
+      p.cap = cap;
+    }
     p.bytes = 2 * p.cap * 8 + p.visited_bytes + 8ull * p.max_layers;
     return p;
   }
@@ -204,6 +241,27 @@ class BatchEngine {
       }
     if (slot < 0) return MGBFS_BATCH_FULL;
     MGBFS_HIP(hipSetDevice(device_));
+    jobs_[slot] = Job{g, visited, max_layers, 1, 0};
+    install(slot, p);
+    *slot_out = slot;
+    return 0;
+  }
+
+  // Per-slot facts: visited set (1 bitmap, 2 hash), retained layers (0 =
+  // all), planned layer capacity and restarts after an outgrown layer.
+  void instance_info(int32_t slot, uint32_t& visited, uint32_t& history, uint64_t& cap, uint32_t& restarts) const {
+    check_slot(slot);
+    const BatchInstance& d = host_inst_[slot];
+    visited = d.visited == kVisBitmap ? 1 : 2;
+    history = d.rolling ? 3 : 0;
+    cap = d.cap;
+    restarts = jobs_[slot].restarts;
+  }
+
+  // Lays out a slot's block (frontiers, layer sizes, visited set) from its
+  // plan and queues the descriptor for installation and seeding.
+  void install(int32_t slot, const BatchPlan& p) {
+    const Graph& g = jobs_[slot].graph;
     Slot& sl = slots_[slot];
     const uint64_t keys_bytes = arena_round(p.cap * 8), sizes_bytes = arena_round(8ull * p.max_layers);
     sl.block = arena_.take(2 * keys_bytes + sizes_bytes + p.visited_bytes, budget_);
@@ -225,7 +283,9 @@ class BatchEngine {
       MGBFS_HIP(fill_u32(d.bitmap, 0u, p.visited_bytes / 4, stream_));
     } else {
       d.table = reinterpret_cast<uint64_t*>(visited_base);
-      d.table_mask = p.visited_bytes / 8 - 1;
+      d.table_slots = p.table_slots;
+      d.table_stride = p.table_slots;
+      d.rolling = p.rolling ? 1u : 0u;
       MGBFS_HIP(fill_u64(d.table, ~0ull, p.visited_bytes / 8, stream_));
     }
     d.tag_shift = 64;
@@ -267,8 +327,6 @@ class BatchEngine {
     kinds_[slot] = g.kind;
     rows_[slot] = g.rows;
     cols_[slot] = g.cols;
-    *slot_out = slot;
-    return 0;
   }
 
   // Runs up to `layers` layers of every active instance; returns how many
@@ -280,9 +338,29 @@ class BatchEngine {
     for (uint32_t k = 0; k < launches; ++k) MGBFS_HIP(hipGraphLaunch(exec_, stream_));
     refresh();
     uint32_t active = 0;
-    for (uint32_t s = 0; s < slots_n_; ++s)
-      if (slots_[s].used && (host_inst_[s].status & kInstActive)) ++active;
+    for (uint32_t s = 0; s < slots_n_; ++s) {
+      if (!slots_[s].used) continue;
+      if (host_inst_[s].status & kInstDone) restart_if_outgrown(s);
+      if ((host_inst_[s].status & kInstActive) || pending_[s] == kInstallSeed) ++active;
+    }
     return active;
+  }
+
+  // A rolling instance whose layer outgrew its planned capacity (next
+  // frontier or table full) starts over with twice the room, up to the
+  // state bound, when the budget allows; otherwise its error stands.
+  void restart_if_outgrown(uint32_t s) {
+    const BatchInstance& d = host_inst_[s];
+    if (!d.rolling || !(d.error & (kErrFutureCap | kErrTableFull)) || slots_[s].reported) return;
+    Job& job = jobs_[s];
+    if (d.cap >= plan(job.graph, job.visited, job.max_layers, job.scale).bound) return;
+    const BatchPlan p = plan(job.graph, job.visited, job.max_layers, job.scale * 2);
+    if (used_ - slots_[s].planned + p.bytes > budget_) return;
+    job.scale *= 2;
+    ++job.restarts;
+    arena_.give(slots_[s].block, budget_);
+    used_ -= slots_[s].planned;
+    install(static_cast<int32_t>(s), p);
   }
 
   void run_all() {
@@ -346,6 +424,7 @@ class BatchEngine {
     used_ -= sl.planned;
     sl = Slot{};
     finished_[slot] = Finished{};
+    jobs_[slot] = Job{};
   }
 
   uint64_t used() const { return used_; }
@@ -364,8 +443,9 @@ class BatchEngine {
     MGBFS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     MGBFS_HIP(batch_reset_flag(flag, stream_));
     for (uint32_t k = 0; k < layers_per_graph_; ++k) {
-      MGBFS_HIP(batch_tiles(inst, slots_n_, kTile, prefix, stream_));
-      MGBFS_HIP(batch_expand(inst, slots_n_, prefix, kTile, grid_, 256, stream_));
+      MGBFS_HIP(batch_tiles(inst, slots_n_, 2 * threads_, prefix, stream_));
+      MGBFS_HIP(batch_clear(inst, slots_n_, prefix, clear_grid_, stream_));
+      MGBFS_HIP(batch_expand(inst, slots_n_, prefix, 2 * threads_, grid_, threads_, stream_));
       MGBFS_HIP(batch_finish(inst, slots_n_, flag, stream_));
     }
     MGBFS_HIP(hipStreamEndCapture(stream_, &graph_));
@@ -444,9 +524,17 @@ class BatchEngine {
     bool ready = false;
   };
 
+  struct Job {
+    Graph graph;
+    uint32_t visited = 0, max_layers = 0, scale = 1, restarts = 0;
+  };
+
   int device_ = 0;
   hipStream_t stream_ = nullptr;
-  uint32_t slots_n_ = 0, grid_ = 0, layers_per_graph_ = kLayersPerGraph;
+  uint32_t slots_n_ = 0, grid_ = 0, clear_grid_ = 0, layers_per_graph_ = kLayersPerGraph;
+  uint32_t threads_ = 256, hash_load_pct_ = 50, layer_pct_ = 30, bitmap_bias_permille_ = 1000;
+  bool full_history_ = false;
+  std::vector<Job> jobs_;
   uint64_t budget_ = 0, used_ = 0;
   DeviceBuffer inst_, prefix_, flag_;
   uint32_t* host_flag_ = nullptr;
@@ -547,6 +635,21 @@ extern "C" int mgbfs_batch_poll(void* batch, int32_t* slots, uint32_t capacity, 
     const std::vector<int32_t> done = static_cast<BatchEngine*>(batch)->poll(capacity);
     *count = static_cast<uint32_t>(done.size());
     for (size_t k = 0; k < done.size(); ++k) slots[k] = done[k];
+    return 0;
+  })
+}
+
+extern "C" int mgbfs_batch_instance_info(void* batch, int32_t slot, uint32_t* visited, uint32_t* history_layers,
+                                         uint64_t* layer_capacity, uint32_t* restarts) {
+  if (!batch) return hipErrorInvalidValue;
+  MGBFS_BATCH_GUARD({
+    uint32_t v = 0, h = 0, r = 0;
+    uint64_t c = 0;
+    static_cast<BatchEngine*>(batch)->instance_info(slot, v, h, c, r);
+    if (visited) *visited = v;
+    if (history_layers) *history_layers = h;
+    if (layer_capacity) *layer_capacity = c;
+    if (restarts) *restarts = r;
     return 0;
   })
 }

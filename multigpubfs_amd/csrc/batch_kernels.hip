@@ -1,8 +1,11 @@
 // Batched multi-instance BFS kernels.  One layer of every active instance
-// is three launches, all capturable in a hipGraph (no host readback, no
+// is four launches, all capturable in a hipGraph (no host readback, no
 // memset nodes):
 //   batch_tiles  : one block scans the per-instance tile counts
-//                  ceil(count / tile) of the active instances -> prefix.
+//                  ceil(count / tile) of the active instances -> prefix,
+//                  and the clear work of rolling instances' next tables.
+//   batch_clear  : empties the table each rolling instance's next layer
+//                  is claimed in (it held layer depth - 2).
 //   batch_expand : a persistent grid walks the global tile index space;
 //                  tile t belongs to the instance i with prefix[i] <= t <
 //                  prefix[i+1].  The instance descriptor (generator tables
@@ -20,9 +23,10 @@ namespace mgbfs {
 
 namespace {
 
-constexpr int kBatchThreads = 256;
-constexpr int kBatchItems = 2;  // parents per thread per tile (tile = 512 parents)
+constexpr int kBatchItems = 2;  // parents per thread per tile (tile = 2 x threads parents)
 constexpr uint32_t kTileRun = 16;  // consecutive tiles one block takes in turn
+constexpr uint32_t kClearChunk = 8192;   // table slots per clear work item
+constexpr uint64_t kRollingProbes = 512;  // probe bound of rolling tables (overflow -> restart)
 
 __device__ __forceinline__ int64_t pymod64(int64_t x, int64_t m) {
   int64_t r = x % m;
@@ -107,6 +111,43 @@ __device__ __forceinline__ uint32_t batch_claim_bitmap(const BatchInstance& s, u
   return claimed;
 }
 
+__device__ __forceinline__ uint64_t home_slot(uint64_t key, uint64_t slots) { return __umul64hi(mix64(key), slots); }
+
+// Rolling tables never hold a run longer than kRollingProbes while their
+// layer fits its capacity (load <= hash_load_pct); a longer run means the
+// layer outgrew it, and the claim fails with kErrTableFull so the instance
+// restarts with more room.  A lookup that reaches the bound has seen every
+// slot a claim of `key` could have used.
+__device__ __forceinline__ bool table_has(const uint64_t* t, uint64_t slots, uint64_t key, uint64_t limit) {
+  uint64_t slot = home_slot(key, slots);
+  for (uint64_t probe = 0; probe < limit; ++probe) {
+    const uint64_t seen = t[slot];
+    if (seen == key) return true;
+    if (seen == ~0ull) return false;
+    slot = slot + 1 == slots ? 0 : slot + 1;
+  }
+  return false;
+}
+
+// Claim-then-compare open addressing: true when this thread inserted `key`
+// (at most `limit` probes).
+__device__ __forceinline__ bool table_claim(uint64_t* t, uint64_t slots, uint64_t key, uint32_t& err, uint64_t limit) {
+  uint64_t slot = home_slot(key, slots);
+  for (uint64_t probe = 0; probe < limit; ++probe) {
+    const uint64_t seen = t[slot];
+    if (seen == key) return false;
+    if (seen == ~0ull) {
+      const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(t + slot), ~0ull,
+                                     static_cast<unsigned long long>(key));
+      if (old == ~0ull) return true;
+      if (old == key) return false;
+    }
+    slot = slot + 1 == slots ? 0 : slot + 1;
+  }
+  err |= kErrTableFull;
+  return false;
+}
+
 // Returns true when this thread claimed `key` (first visit).
 __device__ __forceinline__ bool batch_claim(const BatchInstance& s, uint64_t key, uint32_t& err) {
   if (s.visited == kVisBitmap) {
@@ -121,32 +162,46 @@ __device__ __forceinline__ bool batch_claim(const BatchInstance& s, uint64_t key
     if (*word & bit) return false;
     return !(__hip_atomic_fetch_or(word, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit);
   }
-  uint64_t slot = mix64(key) & s.table_mask;
-  for (uint64_t probe = 0; probe <= s.table_mask; ++probe, slot = (slot + 1) & s.table_mask) {
-    const uint64_t seen = s.table[slot];
-    if (seen == key) return false;
-    if (seen != ~0ull) continue;
-    const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(s.table + slot), ~0ull,
-                                   static_cast<unsigned long long>(key));
-    if (old == ~0ull) return true;
-    if (old == key) return false;
+  if (s.rolling) {
+    const uint32_t d = s.depth % 3;
+    const uint64_t limit = s.table_slots < kRollingProbes ? s.table_slots : kRollingProbes;
+    if (table_has(s.table + ((d + 2) % 3) * s.table_stride, s.table_slots, key, limit) ||
+        table_has(s.table + d * s.table_stride, s.table_slots, key, limit))
+      return false;
+    return table_claim(s.table + ((d + 1) % 3) * s.table_stride, s.table_slots, key, err, limit);
   }
-  err |= kErrTableFull;
-  return false;
+  return table_claim(s.table, s.table_slots, key, err, s.table_slots);
 }
 
 __global__ void batch_reset_flag_kernel(uint32_t* flag) {
   if (threadIdx.x == 0) *flag = 0;
 }
 
-__global__ __launch_bounds__(1024) void batch_tiles_kernel(const BatchInstance* inst, uint32_t slots, uint32_t tile,
-                                                           uint32_t* prefix) {
-  __shared__ uint32_t wave_sums[1024 / kWave];
-  const uint32_t i = threadIdx.x;
-  uint32_t v = 0;
-  if (i < slots && (inst[i].status & kInstActive) && !inst[i].error)
-    v = static_cast<uint32_t>((inst[i].count + tile - 1) / tile);
-  const uint32_t lane = i & (kWave - 1), wave = i / kWave;
+// Chunk t of the clear prefix: the instance with cpre[i] <= t < cpre[i+1]
+// clears slots [k, k + kClearChunk) of table (depth + 1) % 3, k = (t -
+// cpre[i]) * kClearChunk.  Runs before the expansion of the same layer.
+__global__ __launch_bounds__(256) void batch_clear_kernel(const BatchInstance* inst, uint32_t slots,
+                                                          const uint32_t* __restrict__ cpre) {
+  const uint32_t total = cpre[slots];
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    uint32_t lo = 0, hi = slots - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (cpre[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchInstance& s = inst[lo];
+    uint64_t* table = s.table + ((s.depth + 1) % 3) * s.table_stride;
+    const uint64_t k0 = static_cast<uint64_t>(t - cpre[lo]) * kClearChunk;
+    const uint64_t k1 = k0 + kClearChunk < s.table_stride ? k0 + kClearChunk : s.table_stride;
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) table[k] = ~0ull;
+  }
+}
+
+// Exclusive scan of v over the first `slots` threads of a 1024-thread
+// block: out[i] = sum of v below i, out[slots] = total.
+__device__ __forceinline__ void block_scan_out(uint32_t v, uint32_t slots, uint32_t* wave_sums, uint32_t* out) {
+  const uint32_t i = threadIdx.x, lane = i & (kWave - 1), wave = i / kWave;
   uint32_t incl = v;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
@@ -160,21 +215,40 @@ __global__ __launch_bounds__(1024) void batch_tiles_kernel(const BatchInstance* 
     if (w < wave) before += wave_sums[w];
     total += wave_sums[w];
   }
-  if (i < slots) prefix[i] = before + incl - v;
-  if (i == 0) prefix[slots] = total;
+  if (i < slots) out[i] = before + incl - v;
+  if (i == 0) out[slots] = total;
+  __syncthreads();  // wave_sums reuse
+}
+
+// prefix[0..slots]: expansion tiles of the active instances; prefix[slots +
+// 1 ..]: clear chunks (kClearChunk slots) of the table each active rolling
+// instance claims its next layer in (table (depth + 1) % 3, from depth 2).
+__global__ __launch_bounds__(1024) void batch_tiles_kernel(const BatchInstance* inst, uint32_t slots, uint32_t tile,
+                                                           uint32_t* prefix) {
+  __shared__ uint32_t wave_sums[1024 / kWave];
+  const uint32_t i = threadIdx.x;
+  uint32_t v = 0, c = 0;
+  if (i < slots && (inst[i].status & kInstActive) && !inst[i].error) {
+    v = static_cast<uint32_t>((inst[i].count + tile - 1) / tile);
+    if (inst[i].rolling && inst[i].depth >= 2)
+      c = static_cast<uint32_t>((inst[i].table_stride + kClearChunk - 1) / kClearChunk);
+  }
+  block_scan_out(v, slots, wave_sums, prefix);
+  block_scan_out(c, slots, wave_sums, prefix + slots + 1);
 }
 
 // Eight waves per SIMD (64 VGPRs, no spills): the grid of 8 blocks per CU
 // is resident at once and the probes' latency is hidden across waves.
-__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8))) void batch_expand_kernel(BatchInstance* inst, uint32_t slots,
-                                                                     const uint32_t* __restrict__ prefix,
-                                                                     uint32_t tile) {
+template <int kBatchThreads>
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8))) void batch_expand_kernel(
+    BatchInstance* inst, uint32_t slots, const uint32_t* __restrict__ prefix, uint32_t tile) {
   __shared__ uint32_t spre[kBatchMaxSlots + 1];
   __shared__ BatchInstance si;
   __shared__ uint64_t parents[kBatchItems][kBatchThreads];
   __shared__ uint32_t wave_sums[kBatchThreads / kWave];
   __shared__ unsigned long long wg_base;
   __shared__ int staged;
+  __shared__ uint32_t tile_error[2];  // alternating per tile: a slot is rewritten only after a barrier
   for (uint32_t k = threadIdx.x; k <= slots; k += kBatchThreads) spre[k] = prefix[k];
   if (threadIdx.x == 0) staged = -1;
   __syncthreads();
@@ -184,7 +258,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
   // block still samples all instances of the layer.
   // Short layers take one tile per block so every block has work.
   const uint32_t run = min(kTileRun, max(1u, total / gridDim.x));
-  uint32_t i = 0;
+  uint32_t i = 0, phase = 0;
   for (uint32_t t0 = blockIdx.x * run; t0 < total; t0 += gridDim.x * run)
   for (uint32_t t = t0; t < t0 + run && t < total; ++t) {
     // Instance owning tile t: the last i with spre[i] <= t.
@@ -205,6 +279,13 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
       if (threadIdx.x == 0) staged = static_cast<int>(i);
       __syncthreads();
     }
+    // An instance with an error bit is retired by the finish kernel; its
+    // remaining tiles are skipped (a rolling instance that outgrew its
+    // tables stops filling them).
+    if (threadIdx.x == 0)
+      tile_error[phase & 1] = __hip_atomic_load(&inst[i].error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (tile_error[phase++ & 1]) continue;
     const uint64_t p0 = static_cast<uint64_t>(t - spre[i]) * tile;
     const uint64_t p1 = p0 + tile < si.count ? p0 + tile : si.count;
     const uint32_t G = si.generators;
@@ -303,7 +384,9 @@ __global__ void batch_gather_sizes_kernel(const BatchInstance* inst, const uint3
 
 __device__ void seed_instance(BatchInstance& s, uint64_t start_key) {
   uint32_t err = 0;
-  if (!batch_claim(s, start_key, err)) err |= kErrDomain;
+  const bool claimed = s.visited == kVisHash && s.rolling ? table_claim(s.table, s.table_slots, start_key, err, s.table_slots)
+                                                          : batch_claim(s, start_key, err);
+  if (!claimed) err |= kErrDomain;
   s.front[0] = start_key;
   s.count = 1;
   s.next_count = 0;
@@ -344,10 +427,28 @@ hipError_t batch_tiles(BatchInstance* inst, uint32_t slots, uint32_t tile, uint3
 }
 
 hipError_t batch_expand(BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t tile, uint32_t grid,
-                        uint32_t, hipStream_t stream) {
-  if (slots == 0 || slots > kBatchMaxSlots || tile != kBatchThreads * kBatchItems || grid == 0)
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(batch_expand_kernel, dim3(grid), dim3(kBatchThreads), 0, stream, inst, slots, prefix, tile);
+                        uint32_t threads, hipStream_t stream) {
+  if (slots == 0 || slots > kBatchMaxSlots || tile != threads * kBatchItems || grid == 0) return hipErrorInvalidValue;
+  switch (threads) {
+    case 128:
+      hipLaunchKernelGGL(batch_expand_kernel<128>, dim3(grid), dim3(128), 0, stream, inst, slots, prefix, tile);
+      break;
+    case 256:
+      hipLaunchKernelGGL(batch_expand_kernel<256>, dim3(grid), dim3(256), 0, stream, inst, slots, prefix, tile);
+      break;
+    case 512:
+      hipLaunchKernelGGL(batch_expand_kernel<512>, dim3(grid), dim3(512), 0, stream, inst, slots, prefix, tile);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_clear(const BatchInstance* inst, uint32_t slots, const uint32_t* prefix, uint32_t grid,
+                       hipStream_t stream) {
+  if (slots == 0 || slots > kBatchMaxSlots || grid == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_clear_kernel, dim3(grid), dim3(256), 0, stream, inst, slots, prefix + slots + 1);
   return hipGetLastError();
 }
 

@@ -74,22 +74,32 @@ def make_sweep(count: int = 256) -> list[dict]:
     return [make_instance(i) for i in range(count)]
 
 
-def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=16, profile=None):
+# BatchEngine options an autotuned profile may set (mgbfs_batch_options).
+ENGINE_KEYS = ("threads", "hash_load_pct", "history", "layer_pct", "bitmap_bias_permille")
+
+
+def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=16, profile=None, **engine):
     """All instances through one BatchEngine; returns (layer tables, seconds, info).
-    `profile` (from autotune.choose_profile) sets the launch shape."""
+    `profile` (from autotune.choose_profile) sets the launch shape and the
+    engine options; keyword arguments (ENGINE_KEYS) override it.  info counts
+    the visited sets and retained-layer modes chosen and the restarts."""
     from ._native import BatchEngine, device_info
     slots = max_slots or len(instances)
     tables = [None] * len(instances)
     grid = layers_per_graph = 0
+    options = {}
     if profile:
         grid = profile["grid_per_cu"] * device_info(device)["compute_units"]
         layers_per_graph = profile["layers_per_graph"]
         visited = profile["visited"]
-    with BatchEngine(device=device, max_slots=slots, grid=grid, layers_per_graph=layers_per_graph) as b:
+        options = {k: profile[k] for k in ENGINE_KEYS if k in profile}
+    options.update(engine)
+    modes = {}
+    restarts = 0
+    with BatchEngine(device=device, max_slots=slots, grid=grid, layers_per_graph=layers_per_graph, **options) as b:
         t0 = time.perf_counter()
         pending = list(range(len(instances)))
         owner = {}
-        choices = {}
         while pending or owner:
             while pending:
                 slot = b.admit(instances[pending[0]], visited=visited)
@@ -101,12 +111,14 @@ def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=1
                 sizes, status, error = b.result(slot)
                 if error:
                     raise RuntimeError(f"BATCH_INSTANCE_ERROR {owner[slot]}: {error}")
+                info = b.instance_info(slot)
+                key = f"{info['visited']}/{'rolling' if info['history_layers'] == 3 else 'full'}"
+                modes[key] = modes.get(key, 0) + 1
+                restarts += info["restarts"]
                 tables[owner.pop(slot)] = sizes
                 b.release(slot)
         seconds = time.perf_counter() - t0
-        for idx in range(min(len(instances), 4)):
-            choices[idx] = b.plan(instances[idx], visited=visited)[1]
-    return tables, seconds, {"visited_sample": choices}
+    return tables, seconds, {"modes": modes, "restarts": restarts}
 
 
 def run_sequential(instances, *, device=0, backend="permutation_bitmap"):

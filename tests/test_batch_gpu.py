@@ -96,14 +96,90 @@ def test_batch_samples_are_last_layer_states():
 
 
 def test_autotuned_profile_keeps_tables_exact(tmp_path, monkeypatch):
-    from multigpubfs_amd.autotune import CANDIDATES, DEFAULT, choose_profile
+    from multigpubfs_amd.autotune import DEFAULT, DIMENSIONS, choose_profile
     monkeypatch.setenv("MGBFS_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
     subset = GOLDEN[:32]
     inst = [x["graph"] for x in subset]
     profile = choose_profile(inst, pilot=8)
-    chosen = {k: profile[k] for k in DEFAULT}
-    assert chosen == DEFAULT or chosen in CANDIDATES
+    allowed = {k: [DEFAULT[k]] + v for k, v in DIMENSIONS}
+    assert all(profile[k] in allowed[k] for k in DEFAULT)
     assert all(t.get("identical", True) for t in profile["trials"])
+    tried = {k for t in profile["trials"] for k in DEFAULT if t["profile"][k] != DEFAULT[k]}
+    assert {"threads", "grid_per_cu", "layers_per_graph", "visited", "bitmap_bias_permille"} <= tried
     tables, _, _ = run_batched(inst, profile=profile)
     assert tables == [x["layer_sizes"] for x in subset]
     assert choose_profile(inst, pilot=8)["cached"] is True
+
+
+def test_rolling_three_tables_for_undirected_hash_instances():
+    # Forced hash: the inverse-closed half keeps three rolling tables sized
+    # at 30% of the state bound, the directed half the full orbit.
+    subset = GOLDEN[:24]
+    inst = [x["graph"] for x in subset]
+    tables, _, info = run_batched(inst, visited="hash")
+    assert tables == [x["layer_sizes"] for x in subset]
+    closed = sum(GraphDefinition(g).inverse_closed() for g in inst)
+    assert 0 < closed < len(inst)
+    assert info["modes"] == {"hash/rolling": closed, "hash/full": len(inst) - closed}
+    full, _, info = run_batched(inst, visited="hash", history="full")
+    assert full == tables and info["modes"] == {"hash/full": len(inst)}
+
+
+def test_rolling_layer_overflow_restarts_exactly():
+    # Rolling tables sized for 1% of the bound: layers outgrow them, and the
+    # instances restart with doubled room until their largest layer fits.
+    subset = [x for x in GOLDEN[:40] if GraphDefinition(x["graph"]).inverse_closed()][:8]
+    inst = [x["graph"] for x in subset]
+    tables, _, info = run_batched(inst, visited="hash", layer_pct=1)
+    assert tables == [x["layer_sizes"] for x in subset]
+    assert info["restarts"] > 0 and info["modes"] == {"hash/rolling": len(inst)}
+
+
+def test_sl2_rolling_tables_match_oracle_and_report_three_layers():
+    graphs = [sl2(p) for p in (5, 7, 11, 13, 31)]
+    with BatchEngine(max_slots=8) as b:
+        slots = [b.admit(g.to_dict()) for g in graphs]
+        infos = [b.instance_info(s) for s in slots]
+        b.run()
+        results = {s: b.result(s) for s in b.poll()}
+    for g, s, info in zip(graphs, slots, infos):
+        p = g.action["generators"][0]["modulo"]
+        assert info["visited"] == "hash" and info["history_layers"] == 3
+        # 30% of the group order, at least 1024 rows, at most the order.
+        assert info["layer_capacity"] == min(p * (p * p - 1), max(1024, -(-3 * p * (p * p - 1) // 10)))
+        sizes, status, error = results[s]
+        assert error == 0
+        assert sizes == [len(x) for x in O.c_exact_layers(g.to_dict(), p * (p * p - 1))]
+
+
+@pytest.mark.parametrize("threads", [128, 512])
+def test_block_sizes_agree(threads):
+    subset = GOLDEN[:24]
+    tables, _, _ = run_batched([x["graph"] for x in subset], threads=threads)
+    assert tables == [x["layer_sizes"] for x in subset]
+
+
+@pytest.mark.parametrize("load", [25, 75])
+def test_hash_load_factors_agree(load):
+    subset = GOLDEN[:16]
+    tables, _, _ = run_batched([x["graph"] for x in subset], visited="hash", hash_load_pct=load)
+    assert tables == [x["layer_sizes"] for x in subset]
+
+
+def test_bitmap_bias_moves_the_per_instance_choice():
+    # Full history: an S9/S10 bitmap is 1/128 of the hash table's bytes, so
+    # a bias of 5 per mille picks the hash table and 1000 the bitmap.
+    g = GOLDEN[0]["graph"]
+    for bias, want in ((5, "hash"), (1000, "bitmap")):
+        with BatchEngine(max_slots=2, history="full", bitmap_bias_permille=bias) as b:
+            assert b.plan(g)[1] == want
+            slot = b.admit(g)
+            assert b.instance_info(slot)["visited"] == want
+
+
+def test_batch_options_are_validated():
+    for bad in ({"threads": 64}, {"hash_load_pct": 95}, {"layer_pct": 101}):
+        with pytest.raises(RuntimeError):
+            BatchEngine(max_slots=2, **bad)
+    with pytest.raises(ValueError):
+        BatchEngine(max_slots=2, history="three")
