@@ -293,9 +293,10 @@ typedef struct mgbfs_batch_options {
   uint32_t grid;             /* persistent expand grid (0 = 8 x CUs)            */
   uint32_t threads;          /* expand workgroup: 128, 256 or 512 (0 = 256)     */
   uint32_t hash_load_pct;    /* hash tables filled to at most this % (0 = 50)   */
-  uint32_t history;          /* 0: three rolling tables for inverse-closed hash
-                                instances, full history otherwise; 1: full
-                                history for every instance                     */
+  uint32_t history;          /* inverse-closed hash instances: 0 = full history
+                                when it fits the free budget, else three
+                                rolling tables; 1 = always full history;
+                                2 = always three rolling tables                */
   uint32_t layer_pct;        /* rolling instances: layer capacity as % of the
                                 state bound (0 = 30); a layer that outgrows it
                                 restarts the instance with twice the room      */

@@ -228,6 +228,9 @@ def device_info(device: int = 0) -> dict:
                 free_bytes=fr.value)
 
 
+HISTORY = {"auto": 0, "full": 1, "rolling": 2}
+
+
 class BatchOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_slots", C.c_uint32), ("vram_budget", C.c_uint64),
                 ("layers_per_graph", C.c_uint32), ("grid", C.c_uint32), ("threads", C.c_uint32),
@@ -329,14 +332,15 @@ class BatchEngine:
 
     def __init__(self, *, device=0, max_slots=256, vram_budget=0, layers_per_graph=0, grid=0, threads=0,
                  hash_load_pct=0, history="auto", layer_pct=0, bitmap_bias_permille=0):
-        """history: "auto" (three rolling tables for inverse-closed hash
-        instances, full history otherwise) or "full"; zeros take the
-        engine's defaults (mgbfs_batch_options in include/mgbfs_hip.h)."""
+        """history of inverse-closed hash instances: "auto" (full history
+        when it fits the free budget, else three rolling tables), "full" or
+        "rolling"; zeros take the engine's defaults (mgbfs_batch_options in
+        include/mgbfs_hip.h)."""
         self._lib = _bind_batch(load())
-        if history not in ("auto", "full"):
+        if history not in HISTORY:
             raise ValueError("BATCH_OPTION_HISTORY")
         opts = BatchOptions(device, max_slots, vram_budget, layers_per_graph, grid, threads, hash_load_pct,
-                            1 if history == "full" else 0, layer_pct, bitmap_bias_permille)
+                            HISTORY[history], layer_pct, bitmap_bias_permille)
         h = C.c_void_p()
         err = C.create_string_buffer(256)
         if self._lib.mgbfs_batch_create(C.byref(opts), C.byref(h), err, 256):

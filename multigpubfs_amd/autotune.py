@@ -7,8 +7,8 @@ forced), the per-instance bitmap-vs-hash rule (bitmap_bias_permille: an
 instance takes the Lehmer bitmap when its bytes are at most that many
 thousandths of its hash tables'), the hash tables' load factor
 (hash_load_pct), and the retained history of inverse-closed hash instances
-(three rolling tables sized at layer_pct % of the state bound, or the full
-orbit).
+(the full orbit when it fits, or always three rolling tables sized at
+layer_pct % of the state bound).
 
 Profiles are timed on a bounded pilot, a deterministic sample of the
 workload's instances, by coordinate descent: each knob in turn tries its
@@ -38,7 +38,7 @@ DIMENSIONS = [
     ("visited", ["hash"]),
     ("bitmap_bias_permille", [5, 4000]),
     ("hash_load_pct", [30, 70]),
-    ("history", ["full"]),
+    ("history", ["rolling"]),
     ("layer_pct", [50]),
 ]
 HASH_KNOBS = {"hash_load_pct", "history", "layer_pct"}

@@ -124,14 +124,14 @@ class BatchEngine {
     hipDeviceProp_t prop;
     MGBFS_HIP(hipGetDeviceProperties(&prop, device_));
     grid_ = o.grid ? o.grid : static_cast<uint32_t>(prop.multiProcessorCount) * 8;
-    clear_grid_ = static_cast<uint32_t>(prop.multiProcessorCount) * 4;
+    clear_grid_ = static_cast<uint32_t>(prop.multiProcessorCount);
     layers_per_graph_ = o.layers_per_graph ? o.layers_per_graph : kLayersPerGraph;
     threads_ = o.threads ? o.threads : 256;
     if (threads_ != 128 && threads_ != 256 && threads_ != 512) throw std::runtime_error("BATCH_OPTION_THREADS");
     hash_load_pct_ = o.hash_load_pct ? o.hash_load_pct : 50;
     if (hash_load_pct_ < 10 || hash_load_pct_ > 90) throw std::runtime_error("BATCH_OPTION_HASH_LOAD");
-    if (o.history > 1) throw std::runtime_error("BATCH_OPTION_HISTORY");
-    full_history_ = o.history == 1;
+    if (o.history > 2) throw std::runtime_error("BATCH_OPTION_HISTORY");
+    history_ = o.history;
     layer_pct_ = o.layer_pct ? o.layer_pct : 30;
     if (layer_pct_ > 100) throw std::runtime_error("BATCH_OPTION_LAYER_PCT");
     bitmap_bias_permille_ = o.bitmap_bias_permille ? o.bitmap_bias_permille : 1000;
@@ -170,14 +170,25 @@ class BatchEngine {
   }
 
   // Planned footprint (the admission unit) and the visited-set choice.
-  // Hash instances of inverse-closed graphs keep three rolling tables
+  // Hash instances of inverse-closed graphs may keep three rolling tables
   // (history_layers 3) sized, like their frontiers, for layers of up to
-  // layer_pct % of the state bound, times `scale` after restarts; others
-  // keep one table for the whole orbit.  Tables hold at most hash_load_pct %
-  // of their slots.  The automatic visited choice takes the Lehmer bitmap
-  // when its bytes are at most bitmap_bias_permille / 1000 of the hash
-  // tables'.
+  // layer_pct % of the state bound, times `scale` after restarts; otherwise
+  // one table holds the whole orbit.  History policy 2 always rolls them,
+  // 1 never does, and 0 rolls them only when the full-history footprint
+  // does not fit the free budget (three extra lookups per child buy the
+  // memory).  Tables hold at most hash_load_pct % of their slots.  The
+  // automatic visited choice takes the Lehmer bitmap when its bytes are at
+  // most bitmap_bias_permille / 1000 of the hash tables'.
   BatchPlan plan(const Graph& g, uint32_t visited, uint32_t max_layers, uint32_t scale = 1) const {
+    if (history_ == 0 && g.inverse_closed) {
+      const BatchPlan full = plan_with(g, visited, max_layers, false, 1);
+      if (full.visited == kVisBitmap || used_ + full.bytes <= budget_) return full;
+      return plan_with(g, visited, max_layers, true, scale);
+    }
+    return plan_with(g, visited, max_layers, history_ == 2 && g.inverse_closed, scale);
+  }
+
+  BatchPlan plan_with(const Graph& g, uint32_t visited, uint32_t max_layers, bool rolling, uint32_t scale) const {
     BatchPlan p;
     p.max_layers = max_layers ? max_layers : kDefaultMaxLayers;
     if (g.kind == 0) {
@@ -201,7 +212,6 @@ class BatchEngine {
     if (!p.bound || p.bound > (1ull << 33)) throw std::runtime_error("BATCH_STATE_BOUND");
     const bool bitmap_ok = g.kind == 0 && g.distinct_perm() && factorial_u64(g.elements) <= (1ull << 34);
     const uint64_t bitmap_bytes = bitmap_ok ? (factorial_u64(g.elements) + 31) / 32 * 4 : ~0ull;
-    const bool rolling = !full_history_ && g.inverse_closed;
     uint64_t cap = p.bound;
     if (rolling) {
       const uint64_t want = (p.bound * layer_pct_ + 99) / 100 * scale;
@@ -353,8 +363,8 @@ class BatchEngine {
     const BatchInstance& d = host_inst_[s];
     if (!d.rolling || !(d.error & (kErrFutureCap | kErrTableFull)) || slots_[s].reported) return;
     Job& job = jobs_[s];
-    if (d.cap >= plan(job.graph, job.visited, job.max_layers, job.scale).bound) return;
-    const BatchPlan p = plan(job.graph, job.visited, job.max_layers, job.scale * 2);
+    if (d.cap >= state_space_bound(job.graph)) return;
+    const BatchPlan p = plan_with(job.graph, job.visited, job.max_layers, true, job.scale * 2);
     if (used_ - slots_[s].planned + p.bytes > budget_) return;
     job.scale *= 2;
     ++job.restarts;
@@ -533,7 +543,7 @@ class BatchEngine {
   hipStream_t stream_ = nullptr;
   uint32_t slots_n_ = 0, grid_ = 0, clear_grid_ = 0, layers_per_graph_ = kLayersPerGraph;
   uint32_t threads_ = 256, hash_load_pct_ = 50, layer_pct_ = 30, bitmap_bias_permille_ = 1000;
-  bool full_history_ = false;
+  uint32_t history_ = 0;
   std::vector<Job> jobs_;
   uint64_t budget_ = 0, used_ = 0;
   DeviceBuffer inst_, prefix_, flag_;

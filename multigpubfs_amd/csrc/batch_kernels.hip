@@ -27,18 +27,33 @@ constexpr int kBatchItems = 2;  // parents per thread per tile (tile = 2 x threa
 constexpr uint32_t kTileRun = 16;  // consecutive tiles one block takes in turn
 constexpr uint32_t kClearChunk = 8192;   // table slots per clear work item
 constexpr uint64_t kRollingProbes = 512;  // probe bound of rolling tables (overflow -> restart)
+constexpr uint32_t kErrorCheckTiles = 8;  // tiles between reads of a rolling instance's error bits
 
 __device__ __forceinline__ int64_t pymod64(int64_t x, int64_t m) {
   int64_t r = x % m;
   return r < 0 ? r + m : r;
 }
 
+// Device pointers read out of descriptors (in LDS or global memory) carry no
+// address space, so accesses through them compile to FLAT instructions; a
+// FLAT access also counts against the LDS wait counter, so every wait for an
+// LDS read would wait for the bitmap and table probes in flight as well.
+// These casts make them global accesses.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ global_ptr<T> gp(T* p) {
+  return (global_ptr<T>)p;
+}
+
 // child nibble i = parent nibble src[i], the sources read from one packed word.
+// The loops over elements stay rolled (n is uniform): the expand kernel's
+// code must fit the instruction cache, and unrolled copies for 16 elements x
+// 16 generators x 2 parents did not (waves stalled on instruction fetch).
 __device__ __forceinline__ uint64_t apply_word(uint64_t key, uint64_t sw, uint32_t n) {
   uint64_t child = 0;
-#pragma unroll
-  for (uint32_t i = 0; i < kMaxDegree; ++i)
-    if (i < n) child |= ((key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u) << (4u * i);
+#pragma unroll 1
+  for (uint32_t i = 0; i < n; ++i) child |= ((key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u) << (4u * i);
   return child;
 }
 
@@ -48,15 +63,13 @@ template <typename R>
 __device__ __forceinline__ R child_rank(uint64_t key, uint64_t sw, uint32_t n, bool& bad) {
   uint32_t seen = 0;
   R r = 0;
-#pragma unroll
-  for (uint32_t i = 0; i < kMaxDegree; ++i) {
-    if (i < n) {
-      const uint32_t v = static_cast<uint32_t>(key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u;
-      bad |= (v >= n) | ((seen >> v) & 1u);
-      const uint32_t d = v - static_cast<uint32_t>(__popc(seen & ((1u << v) - 1u)));
-      seen |= 1u << v;
-      r = r * static_cast<R>(n - i) + static_cast<R>(d);
-    }
+#pragma unroll 1
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t v = static_cast<uint32_t>(key >> (4u * ((sw >> (4u * i)) & 15u))) & 15u;
+    bad |= (v >= n) | ((seen >> v) & 1u);
+    const uint32_t d = v - static_cast<uint32_t>(__popc(seen & ((1u << v) - 1u)));
+    seen |= 1u << v;
+    r = r * static_cast<R>(n - i) + static_cast<R>(d);
   }
   return r;
 }
@@ -78,35 +91,42 @@ __device__ __forceinline__ uint64_t batch_apply(const BatchInstance& s, uint64_t
   return child;
 }
 
-// Bitmap claims of every child of one parent (generators other than the
-// one its tag names), degree <= 12 so a rank is one 32-bit word: ranks
-// first, then every bitmap word load, then the atomics for the clear bits,
-// so a thread keeps all of a parent's probes in flight at once.  Returns
-// the claimed generators as a bit mask.
+// Bitmap claims of the children of one parent (generators other than the
+// one its tag names), degree <= 12 so a rank is one 32-bit word.  Generators
+// go in groups of kClaimGroup: the group's ranks, then its bitmap word
+// loads, then the atomics for the clear bits, so a thread keeps a group's
+// probes in flight together.  Returns the claimed generators as a bit mask.
+constexpr uint32_t kClaimGroup = 4;
+
 __device__ __forceinline__ uint32_t batch_claim_bitmap(const BatchInstance& s, uint64_t key, uint32_t& err) {
   constexpr uint32_t kNone = ~0u;  // above every rank (12! < 2^32 - 1)
   const uint32_t G = s.generators, n = s.degree;
   const uint32_t skip = s.tag_shift < 64 ? static_cast<uint32_t>(key >> s.tag_shift) : 0u;
-  uint32_t rank[kBatchMaxGens], seen[kBatchMaxGens];
-#pragma unroll
-  for (uint32_t g = 0; g < kBatchMaxGens; ++g) {
-    rank[g] = kNone;
-    if (g >= G || g + 1 == skip) continue;
-    bool bad = false;
-    const uint32_t r = child_rank<uint32_t>(key, s.src_word[g], n, bad);
-    if (bad || r >= s.bitmap_bits) err |= kErrDomain;
-    else rank[g] = r;
-  }
-#pragma unroll
-  for (uint32_t g = 0; g < kBatchMaxGens; ++g) seen[g] = rank[g] != kNone ? s.bitmap[rank[g] >> 5] : 0u;
+  const global_ptr<uint32_t> bitmap = gp(s.bitmap);
   uint32_t claimed = 0;
+#pragma unroll 1
+  for (uint32_t g0 = 0; g0 < G; g0 += kClaimGroup) {
+    uint32_t rank[kClaimGroup], seen[kClaimGroup];
 #pragma unroll
-  for (uint32_t g = 0; g < kBatchMaxGens; ++g) {
-    if (rank[g] == kNone) continue;
-    const uint32_t bit = 1u << (rank[g] & 31u);
-    if (!(seen[g] & bit) &&
-        !(__hip_atomic_fetch_or(s.bitmap + (rank[g] >> 5), bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
-      claimed |= 1u << g;
+    for (uint32_t k = 0; k < kClaimGroup; ++k) {
+      const uint32_t g = g0 + k;
+      rank[k] = kNone;
+      if (g >= G || g + 1 == skip) continue;
+      bool bad = false;
+      const uint32_t r = child_rank<uint32_t>(key, s.src_word[g], n, bad);
+      if (bad || r >= s.bitmap_bits) err |= kErrDomain;
+      else rank[k] = r;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kClaimGroup; ++k) seen[k] = rank[k] != kNone ? bitmap[rank[k] >> 5] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kClaimGroup; ++k) {
+      if (rank[k] == kNone) continue;
+      const uint32_t bit = 1u << (rank[k] & 31u);
+      if (!(seen[k] & bit) &&
+          !(__hip_atomic_fetch_or(bitmap + (rank[k] >> 5), bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+        claimed |= 1u << (g0 + k);
+    }
   }
   return claimed;
 }
@@ -118,7 +138,8 @@ __device__ __forceinline__ uint64_t home_slot(uint64_t key, uint64_t slots) { re
 // layer outgrew it, and the claim fails with kErrTableFull so the instance
 // restarts with more room.  A lookup that reaches the bound has seen every
 // slot a claim of `key` could have used.
-__device__ __forceinline__ bool table_has(const uint64_t* t, uint64_t slots, uint64_t key, uint64_t limit) {
+__device__ __forceinline__ bool table_has(const uint64_t* table, uint64_t slots, uint64_t key, uint64_t limit) {
+  const global_ptr<const uint64_t> t = gp(table);
   uint64_t slot = home_slot(key, slots);
   for (uint64_t probe = 0; probe < limit; ++probe) {
     const uint64_t seen = t[slot];
@@ -131,15 +152,18 @@ __device__ __forceinline__ bool table_has(const uint64_t* t, uint64_t slots, uin
 
 // Claim-then-compare open addressing: true when this thread inserted `key`
 // (at most `limit` probes).
-__device__ __forceinline__ bool table_claim(uint64_t* t, uint64_t slots, uint64_t key, uint32_t& err, uint64_t limit) {
+__device__ __forceinline__ bool table_claim(uint64_t* table, uint64_t slots, uint64_t key, uint32_t& err,
+                                            uint64_t limit) {
+  const global_ptr<uint64_t> t = gp(table);
   uint64_t slot = home_slot(key, slots);
   for (uint64_t probe = 0; probe < limit; ++probe) {
     const uint64_t seen = t[slot];
     if (seen == key) return false;
     if (seen == ~0ull) {
-      const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(t + slot), ~0ull,
-                                     static_cast<unsigned long long>(key));
-      if (old == ~0ull) return true;
+      uint64_t old = ~0ull;
+      if (__hip_atomic_compare_exchange_strong(t + slot, &old, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+        return true;
       if (old == key) return false;
     }
     slot = slot + 1 == slots ? 0 : slot + 1;
@@ -157,7 +181,7 @@ __device__ __forceinline__ bool batch_claim(const BatchInstance& s, uint64_t key
       err |= kErrDomain;
       return false;
     }
-    uint32_t* word = s.bitmap + (r >> 5);
+    const global_ptr<uint32_t> word = gp(s.bitmap) + (r >> 5);
     const uint32_t bit = 1u << (r & 31u);
     if (*word & bit) return false;
     return !(__hip_atomic_fetch_or(word, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit);
@@ -191,7 +215,7 @@ __global__ __launch_bounds__(256) void batch_clear_kernel(const BatchInstance* i
       else hi = mid - 1;
     }
     const BatchInstance& s = inst[lo];
-    uint64_t* table = s.table + ((s.depth + 1) % 3) * s.table_stride;
+    const global_ptr<uint64_t> table = gp(s.table) + ((s.depth + 1) % 3) * s.table_stride;
     const uint64_t k0 = static_cast<uint64_t>(t - cpre[lo]) * kClearChunk;
     const uint64_t k1 = k0 + kClearChunk < s.table_stride ? k0 + kClearChunk : s.table_stride;
     for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) table[k] = ~0ull;
@@ -258,7 +282,8 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
   // block still samples all instances of the layer.
   // Short layers take one tile per block so every block has work.
   const uint32_t run = min(kTileRun, max(1u, total / gridDim.x));
-  uint32_t i = 0, phase = 0;
+  uint32_t i = 0, phase = 0, since_check = 0;
+  bool retired = false;
   for (uint32_t t0 = blockIdx.x * run; t0 < total; t0 += gridDim.x * run)
   for (uint32_t t = t0; t < t0 + run && t < total; ++t) {
     // Instance owning tile t: the last i with spre[i] <= t.
@@ -278,14 +303,23 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
       for (uint32_t w = threadIdx.x; w < sizeof(BatchInstance) / 4; w += kBatchThreads) dst[w] = src[w];
       if (threadIdx.x == 0) staged = static_cast<int>(i);
       __syncthreads();
+      since_check = kErrorCheckTiles;
     }
-    // An instance with an error bit is retired by the finish kernel; its
-    // remaining tiles are skipped (a rolling instance that outgrew its
-    // tables stops filling them).
-    if (threadIdx.x == 0)
-      tile_error[phase & 1] = __hip_atomic_load(&inst[i].error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (tile_error[phase++ & 1]) continue;
+    // A rolling instance that outgrew its tables has an error bit set (it is
+    // retired by the finish kernel and restarted by the host); its remaining
+    // tiles are skipped so the full tables are not probed further.  The bit
+    // is read once per kErrorCheckTiles tiles: a read and barrier per tile
+    // doubled the expansion time of large bitmap instances.
+    if (si.rolling) {
+      if (since_check++ >= kErrorCheckTiles - 1) {
+        since_check = 0;
+        if (threadIdx.x == 0)
+          tile_error[phase & 1] = __hip_atomic_load(&inst[i].error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        retired = tile_error[phase++ & 1] != 0;
+      }
+      if (retired) continue;
+    }
     const uint64_t p0 = static_cast<uint64_t>(t - spre[i]) * tile;
     const uint64_t p1 = p0 + tile < si.count ? p0 + tile : si.count;
     const uint32_t G = si.generators;
@@ -294,7 +328,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
     for (int it = 0; it < kBatchItems; ++it) {
       const uint64_t p = p0 + static_cast<uint64_t>(it) * kBatchThreads + threadIdx.x;
       const bool valid = p < p1;
-      const uint64_t key = valid ? si.front[p] : 0;
+      const uint64_t key = valid ? gp(si.front)[p] : 0;
       parents[it][threadIdx.x] = key;
       if (!valid) continue;
       if (si.visited == kVisBitmap && si.degree <= 12) {
@@ -334,7 +368,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(8
           m &= m - 1;
           const uint32_t g = b % kBatchMaxGens;
           const uint64_t tag = si.tag_shift < 64 ? static_cast<uint64_t>(si.child_tag[g]) << si.tag_shift : 0ull;
-          si.next[pos++] = batch_apply(si, parents[b / kBatchMaxGens][threadIdx.x], g) | tag;
+          gp(si.next)[pos++] = batch_apply(si, parents[b / kBatchMaxGens][threadIdx.x], g) | tag;
         }
       }
     }
@@ -376,7 +410,7 @@ __global__ void batch_finish_kernel(BatchInstance* inst, uint32_t slots, uint32_
 // (slot, offset, count) and copies layer_sizes[0, count) to out[offset...].
 __global__ void batch_gather_sizes_kernel(const BatchInstance* inst, const uint32_t* req, uint32_t n, uint64_t* out) {
   for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
-    const uint64_t* sizes = inst[req[3 * q]].layer_sizes;
+    const global_ptr<const uint64_t> sizes = gp(static_cast<const uint64_t*>(inst[req[3 * q]].layer_sizes));
     const uint32_t off = req[3 * q + 1], count = req[3 * q + 2];
     for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) out[off + k] = sizes[k];
   }

@@ -116,13 +116,33 @@ def test_rolling_three_tables_for_undirected_hash_instances():
     # at 30% of the state bound, the directed half the full orbit.
     subset = GOLDEN[:24]
     inst = [x["graph"] for x in subset]
-    tables, _, info = run_batched(inst, visited="hash")
+    tables, _, info = run_batched(inst, visited="hash", history="rolling")
     assert tables == [x["layer_sizes"] for x in subset]
     closed = sum(GraphDefinition(g).inverse_closed() for g in inst)
     assert 0 < closed < len(inst)
     assert info["modes"] == {"hash/rolling": closed, "hash/full": len(inst) - closed}
     full, _, info = run_batched(inst, visited="hash", history="full")
     assert full == tables and info["modes"] == {"hash/full": len(inst)}
+    auto, _, info = run_batched(inst, visited="hash")  # the full orbit fits: no rolling
+    assert auto == tables and info["modes"] == {"hash/full": len(inst)}
+
+
+def test_rolling_tables_when_full_history_does_not_fit():
+    # A budget below the full-history footprint but above the rolling one:
+    # the automatic policy admits the instance with three rolling tables.
+    g = sl2(31).to_dict()
+    with BatchEngine(max_slots=2, history="full") as b:
+        full_bytes = b.plan(g)[0]
+    with BatchEngine(max_slots=2, history="rolling") as b:
+        rolling_bytes = b.plan(g)[0]
+    assert rolling_bytes < full_bytes
+    with BatchEngine(max_slots=2, vram_budget=(rolling_bytes + full_bytes) // 2) as b:
+        slot = b.admit(g)
+        assert b.instance_info(slot)["history_layers"] == 3
+        b.run()
+        (done,) = b.poll()
+        sizes, status, error = b.result(done)
+    assert error == 0 and sizes == [len(x) for x in O.c_exact_layers(g, 31 * (31 * 31 - 1))]
 
 
 def test_rolling_layer_overflow_restarts_exactly():
@@ -130,14 +150,14 @@ def test_rolling_layer_overflow_restarts_exactly():
     # instances restart with doubled room until their largest layer fits.
     subset = [x for x in GOLDEN[:40] if GraphDefinition(x["graph"]).inverse_closed()][:8]
     inst = [x["graph"] for x in subset]
-    tables, _, info = run_batched(inst, visited="hash", layer_pct=1)
+    tables, _, info = run_batched(inst, visited="hash", history="rolling", layer_pct=1)
     assert tables == [x["layer_sizes"] for x in subset]
     assert info["restarts"] > 0 and info["modes"] == {"hash/rolling": len(inst)}
 
 
 def test_sl2_rolling_tables_match_oracle_and_report_three_layers():
     graphs = [sl2(p) for p in (5, 7, 11, 13, 31)]
-    with BatchEngine(max_slots=8) as b:
+    with BatchEngine(max_slots=8, history="rolling") as b:
         slots = [b.admit(g.to_dict()) for g in graphs]
         infos = [b.instance_info(s) for s in slots]
         b.run()
