@@ -410,6 +410,34 @@ struct BucketBounds {
 };
 
 
+// load_bounds in two steps: the loads, then the arithmetic on their values,
+// for a sub-bucket f < a.buckets with a current bank (and, when kPrev, a
+// previous one).  Unconditional loads: a load under a branch leaves a
+// register copy at the merge, and the copy waits for the load.
+struct RawBounds {
+  uint32_t count, clo, chi, plo, phi;
+};
+
+template <bool kPrev>
+__device__ __forceinline__ RawBounds load_raw_bounds(const UniqueArgs& a, uint32_t f) {
+  const uint64_t n = a.buckets;
+  const uint32_t xlo = static_cast<uint32_t>(((static_cast<uint64_t>(f) << 32) + n - 1) / n);
+  const bool last = f + 1 == n;
+  const uint32_t xhi = last ? 0u : static_cast<uint32_t>(((static_cast<uint64_t>(f + 1) << 32) + n - 1) / n) - 1u;
+  RawBounds r{a.scount[f], a.cur_off[__umulhi(xlo, a.nbo_c)],
+              a.cur_off[last ? a.nbo_c : __umulhi(xhi, a.nbo_c) + 1], 0, 0};
+  if constexpr (kPrev) {
+    r.plo = a.prev_off[__umulhi(xlo, a.nbo_p)];
+    r.phi = a.prev_off[last ? a.nbo_p : __umulhi(xhi, a.nbo_p) + 1];
+  }
+  return r;
+}
+
+__device__ __forceinline__ BucketBounds finish_bounds(uint32_t f, const RawBounds& r) {
+  const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
+  return BucketBounds{s, s + r.count, r.clo, r.chi, r.plo, r.phi};
+}
+
 __device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_t f) {
   BucketBounds b{0, 0, 0, 0, 0, 0};
   if (f >= a.buckets) return b;
@@ -488,6 +516,7 @@ constexpr int kPairsPerThread = kUniqueSlots / 2 / kUniqueThreads;  // table slo
 // banks, marking matched slots -> the occupied, unmarked slots are the
 // survivors: count them per fine bucket, scan, and write them back into
 // the slab grouped by fine bucket.
+template <bool kPrev>
 __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void unique_diff_kernel(
     UniqueArgs a) {
   __shared__ UniqueLds u;
@@ -502,9 +531,20 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
   };
   unsigned long long unique_block = 0, cand_block = 0;
   for (uint32_t f = blockIdx.x; f < a.buckets; f += gridDim.x) {
+    // The bounds' loads go out first, then the slab's, and only then are the
+    // bounds used: waiting for them (to address the bank slice) leaves the
+    // 40 KB slab in flight, so the dependent bank loads go out while it
+    // streams in.
+    RawBounds raw = load_raw_bounds<kPrev>(a, f);
     BucketData dcur;
-    load_slab(a, f, dcur);  // in flight together with the bounds loads
-    const BucketBounds bcur = load_bounds(a, f);
+    load_slab(a, f, dcur);
+    // The empty asm "redefines" the bounds after the slab loads are issued,
+    // so no use of them (and no wait for them) is scheduled above the loads.
+    if constexpr (kPrev)
+      asm volatile("" : "+v"(raw.count), "+v"(raw.clo), "+v"(raw.chi), "+v"(raw.plo), "+v"(raw.phi)::"memory");
+    else
+      asm volatile("" : "+v"(raw.count), "+v"(raw.clo), "+v"(raw.chi)::"memory");
+    const BucketBounds bcur = finish_bounds(f, raw);
     load_bank(a, bcur, dcur);
     const uint64_t s = bcur.s, e = bcur.e;
     const uint64_t c = e - s;
@@ -777,7 +817,7 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t n
                               uint32_t nbo_p, uint32_t fb, uint32_t* fine_counts, uint32_t* error,
                               unsigned long long* unique_total, unsigned long long* cand_total, uint32_t tagbits,
                               bool merge, uint32_t grid_limit, hipStream_t stream) {
-  if (nsub == 0 || fb > kMaxFineBits || (static_cast<uint64_t>(nsub) << fb) > (1ull << kMaxBucketBits) ||
+  if (nsub == 0 || !cur || !cur_off || fb > kMaxFineBits || (static_cast<uint64_t>(nsub) << fb) > (1ull << kMaxBucketBits) ||
       (cur && (nbo_c == 0 || nbo_c > (1u << kMaxBucketBits))) ||
       (prev && (nbo_p == 0 || nbo_p > (1u << kMaxBucketBits))) || grid_limit == 0)
     return hipErrorInvalidValue;
@@ -789,7 +829,10 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t n
   UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, cand_total, tagmask,
                1ull << (tagbits - 1), merge ? 1u : 0u, fb, nbo_c, nbo_p, tagbits, nsub};
   const uint32_t grid = std::min<uint32_t>(nsub, grid_limit);
-  hipLaunchKernelGGL(unique_diff_kernel, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
+  if (prev)
+    hipLaunchKernelGGL(unique_diff_kernel<true>, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL(unique_diff_kernel<false>, dim3(grid), dim3(kUniqueThreads), 0, stream, a);
   return hipGetLastError();
 }
 
