@@ -92,10 +92,10 @@ __device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t c
 
 // Shared tile machinery.  Each thread holds up to PER tile elements in
 // registers (bit k of `valid` marks v[k]); they are ranked within their
-// bucket by LDS atomics and placed bucket-contiguously in t.s.  reserve(b, n)
-// returns the offset of bucket b's run of n elements inside its slab, which
-// starts at slab_base(b) (kDropRun: the slab is full and the run is
-// dropped); the run is then written contiguously.  Run offsets stay 32-bit
+// bucket by LDS atomics and placed bucket-contiguously in t.s.  Bucket b's
+// run of n elements is reserved in its slab (which starts at slab_base(b))
+// through runs.cursor(b); kDropRun: the slab is full and the run is
+// dropped.  The run is then written contiguously.  Run offsets stay 32-bit
 // so two 512-thread tiles fit one CU's LDS.
 constexpr uint32_t kDropRun = ~0u;
 
@@ -116,9 +116,30 @@ struct TileLds {
 // write-out computes 64-bit addresses and skips dropped runs.
 constexpr uint32_t kWideOut = ~0u;
 
-template <int PER, typename BucketFn, typename ReserveFn, typename BaseFn>
+// Run cursors of a routing pass: cursor(b) counts the entries reserved in
+// bucket b's slab; settle(off, n) checks a run reserved at off against the
+// slab capacity (kDropRun and the error bit when it does not fit, or when
+// n = 0).
+struct RunCursors {
+  uint32_t* base;
+  uint64_t stride;  // cursor of bucket b at base[b * stride]
+  uint32_t cap;
+  uint32_t* error;
+  uint32_t err_bit;
+  __device__ __forceinline__ uint32_t* cursor(uint32_t b) const { return base + static_cast<uint64_t>(b) * stride; }
+  __device__ __forceinline__ uint32_t settle(uint32_t off, uint32_t n) const {
+    if (n == 0) return kDropRun;
+    if (static_cast<uint64_t>(off) + n > cap) {
+      atomicOr(error, err_bit);
+      return kDropRun;
+    }
+    return off;
+  }
+};
+
+template <int PER, typename BucketFn, typename BaseFn>
 __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid, uint32_t buckets,
-                                           BucketFn bkt, ReserveFn reserve, BaseFn slab_base,
+                                           BucketFn bkt, const RunCursors& runs, BaseFn slab_base,
                                            uint64_t* __restrict__ out, uint32_t dump) {
   uint32_t rank[PER];
 #pragma unroll
@@ -126,14 +147,23 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   __syncthreads();
   lds_exclusive_scan(t.lhist, t.lstart, buckets, t.wave_sums);
   const uint32_t ne = t.lstart[buckets - 1] + t.lhist[buckets - 1];
-  // Run reservations first: their round trips overlap the LDS placement.
+  // Run reservations first: both atomics go out before the LDS placement
+  // and their offsets are checked after it, so the round trips overlap the
+  // placement.  (A check right after each atomic branches on its result and
+  // waits for it there.)  Buckets without a run issue no atomic: a tile's
+  // children cluster into few buckets.
   static_assert((1u << kMaxPassBits) <= 2 * kRouteThreads, "two buckets per thread");
   const uint32_t b0 = threadIdx.x, b1 = threadIdx.x + kRouteThreads;
-  const uint32_t off0 = b0 < buckets ? reserve(b0, t.lhist[b0]) : kDropRun;
-  const uint32_t off1 = b1 < buckets ? reserve(b1, t.lhist[b1]) : kDropRun;
+  const uint32_t n0 = b0 < buckets ? t.lhist[b0] : 0u, n1 = b1 < buckets ? t.lhist[b1] : 0u;
+  uint32_t off0 = 0, off1 = 0;
+  if (n0) off0 = atomicAdd(runs.cursor(b0), n0);
+  if (n1) off1 = atomicAdd(runs.cursor(b1), n1);
 #pragma unroll
   for (int k = 0; k < PER; ++k)
     if ((valid >> k) & 1u) t.s[t.lstart[bkt(v[k])] + rank[k]] = v[k];
+  asm volatile("" : "+v"(off0), "+v"(off1));  // the offsets are used only from here on
+  off0 = runs.settle(off0, n0);
+  off1 = runs.settle(off1, n1);
   if (dump != kWideOut) {
     auto dest = [&](uint32_t b, uint32_t off) {
       return (off == kDropRun ? dump : static_cast<uint32_t>(slab_base(b)) + off) - t.lstart[b];
@@ -161,19 +191,6 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
   __syncthreads();
-}
-
-// Reserves n entries in the slab counted by *cursor (capacity cap); returns
-// the run's offset in the slab, or kDropRun (and sets err_bit) on overflow.
-__device__ __forceinline__ uint32_t reserve_run(uint32_t* __restrict__ cursor, uint32_t cap, uint32_t n,
-                                                uint32_t* __restrict__ error, uint32_t err_bit) {
-  if (n == 0) return kDropRun;
-  const uint32_t off = atomicAdd(cursor, n);
-  if (static_cast<uint64_t>(off) + n > cap) {
-    atomicOr(error, err_bit);
-    return kDropRun;
-  }
-  return off;
 }
 
 // MASKED: every parent carries a non-empty back-edge mask (all layers after
@@ -207,9 +224,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
   const uint32_t part = blockIdx.x % P;
   for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
   __syncthreads();
-  auto reserve = [=](uint32_t b, uint32_t n) {
-    return reserve_run(&cursors[static_cast<uint64_t>(b) * P + part], cap, n, error, err_bit);
-  };
+  const RunCursors runs{cursors + part, P, cap, error, err_bit};
   auto slab_base = [=](uint32_t b) { return (static_cast<uint64_t>(b) * P + part) * stride; };
   auto bkt = [nsub, K2](uint64_t h) { return range_index(h, nsub) >> K2; };
   // Tiles of contiguous parents go to blocks round-robin (tile i to block
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       }
 #pragma unroll
       for (int k = 0; k < 2 * PP; ++k) w[k] = (keymix(w[k], L.km) << L.tagbits) | wt[k];
-      route_regs(t, w, wvalid, buckets, bkt, reserve, slab_base, out, dump);
+      route_regs(t, w, wvalid, buckets, bkt, runs, slab_base, out, dump);
     } else {
       uint64_t v[T::kRegs];
 #pragma unroll
@@ -293,7 +308,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 #pragma unroll
         for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
       }
-      route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
+      route_regs(t, v, valid, buckets, bkt, runs, slab_base, out, dump);
     }
   }
 }
@@ -319,9 +334,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
   const uint64_t r0 = slab * lcap, r1 = r0 + min(lcount[slab], lcap);
   auto bkt = [nsub, K2](uint64_t h) { return range_index(h, nsub) & ((1u << K2) - 1u); };
   const uint64_t first = static_cast<uint64_t>(blockIdx.x / P) << K2;  // first sub-bucket of this lane
-  auto reserve = [=](uint32_t b, uint32_t n) {
-    return reserve_run(&scount[first + b], kUniqueCap, n, error, kErrBucketCap);
-  };
+  const RunCursors runs{scount + first, 1, kUniqueCap, error, kErrBucketCap};
   auto slab_base = [=](uint32_t b) { return (first + b) * kSlabStride; };
   for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
     uint64_t v[kTilePerThread];
@@ -332,7 +345,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
       v[k] = i < r1 ? in[i] : 0;
       if (i < r1) valid |= 1u << k;
     }
-    route_regs(t, v, valid, buckets, bkt, reserve, slab_base, out, dump);
+    route_regs(t, v, valid, buckets, bkt, runs, slab_base, out, dump);
   }
 }
 
