@@ -21,7 +21,8 @@ template <int G>
 __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
     const uint64_t* __restrict__ frontier, uint64_t count, PackedGenerators gens,
     uint32_t* __restrict__ bitmap, uint64_t bitmap_bits, uint64_t* __restrict__ next,
-    unsigned long long* __restrict__ next_count, uint64_t next_cap, uint32_t* __restrict__ error) {
+    unsigned long long* __restrict__ next_count, uint64_t count_base, uint64_t next_cap,
+    uint32_t* __restrict__ error) {
   constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
   __shared__ uint64_t parents[kItems][kBitmapThreads];
   __shared__ uint32_t wave_sums[kBitmapThreads / kWave];
@@ -40,8 +41,15 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
     parents[it][threadIdx.x] = key;
     if (!valid) continue;
     const uint32_t skip = gens.tag_shift < 64 ? static_cast<uint32_t>(key >> gens.tag_shift) : 0u;
+    // A parent's probes go out together: every child's rank, then every
+    // bitmap word load, then the atomics for the clear bits.
+    uint64_t word[G];
+    uint32_t bit[G], seen[G];
+    uint32_t live = 0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
+      word[g] = 0;
+      bit[g] = 0;
       if (static_cast<uint32_t>(g) + 1 == skip) continue;
       const uint64_t c = apply_groups(key, gens, g);
       bool bad;
@@ -50,14 +58,17 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
         errbits |= kErrDomain;
         continue;
       }
-      uint32_t* word = bitmap + (r >> 5);
-      const uint32_t bit = 1u << (r & 31u);
-      uint32_t seen = *word;
-      if (!(seen & bit)) {
-        seen = __hip_atomic_fetch_or(word, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!(seen & bit)) newmask |= 1u << (it * G + g);
-      }
+      word[g] = r >> 5;
+      bit[g] = 1u << (r & 31u);
+      live |= 1u << g;
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g) seen[g] = (live >> g) & 1u ? bitmap[word[g]] : 0u;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (((live >> g) & 1u) && !(seen[g] & bit[g]) &&
+          !(__hip_atomic_fetch_or(bitmap + word[g], bit[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit[g]))
+        newmask |= 1u << (it * G + g);
   }
 
   // Phase 2: workgroup exclusive scan of per-thread new-child counts and one
@@ -80,7 +91,8 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
     if (w < static_cast<int>(wave)) wave_off += s;
     total += s;
   }
-  if (threadIdx.x == 0) wg_base = total ? atomicAdd(next_count, static_cast<unsigned long long>(total)) : 0ull;
+  if (threadIdx.x == 0)
+    wg_base = total ? atomicAdd(next_count, static_cast<unsigned long long>(total)) - count_base : 0ull;
   __syncthreads();
 
   // Phase 3: write this thread's new children contiguously.
@@ -119,24 +131,25 @@ __global__ void bitmap_seed_kernel(uint32_t degree, uint64_t start_key, uint32_t
 template <int G>
 static hipError_t launch_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens,
                                 uint32_t* bitmap, uint64_t bits, uint64_t* next, unsigned long long* next_count,
-                                uint64_t next_cap, uint32_t* error, hipStream_t stream) {
+                                uint64_t count_base, uint64_t next_cap, uint32_t* error, hipStream_t stream) {
   constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
   const uint64_t per_block = static_cast<uint64_t>(kBitmapThreads) * kItems;
   const uint64_t blocks = (count + per_block - 1) / per_block;
   if (blocks == 0) return hipSuccess;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(bitmap_expand_kernel<G>, dim3(static_cast<uint32_t>(blocks)), dim3(kBitmapThreads), 0, stream,
-                     frontier, count, gens, bitmap, bits, next, next_count, next_cap, error);
+                     frontier, count, gens, bitmap, bits, next, next_count, count_base, next_cap, error);
   return hipGetLastError();
 }
 
 hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens, uint32_t* bitmap,
-                         uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t next_cap,
-                         uint32_t* error, hipStream_t stream) {
+                         uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t count_base,
+                         uint64_t next_cap, uint32_t* error, hipStream_t stream) {
   switch (gens.count) {
-#define MGBFS_G(k) \
-  case k:          \
-    return launch_expand<k>(frontier, count, gens, bitmap, bitmap_bits, next, next_count, next_cap, error, stream);
+#define MGBFS_G(k)                                                                                            \
+  case k:                                                                                                     \
+    return launch_expand<k>(frontier, count, gens, bitmap, bitmap_bits, next, next_count, count_base, next_cap, \
+                            error, stream);
     MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
     MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
 #undef MGBFS_G

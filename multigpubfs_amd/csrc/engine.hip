@@ -139,6 +139,7 @@ class BitmapEngine final : public Engine {
     read_control();
     if (host_ctl_[1]) throw std::runtime_error("BITMAP_START_DOMAIN");
     count_ = 1;
+    counter_base_ = 0;
     prev_count_ = stash_count_ = 0;
     complete_ = false;
     layers.assign(1, 1);
@@ -164,15 +165,18 @@ class BitmapEngine final : public Engine {
     const uint64_t remaining = domain_ - total();
     const uint64_t want = std::min(gens_.count * count_, remaining);
     if (want > next_cap_) grow_next(std::min(cap_limit_, std::max(want, next_cap_ * 2)));
-    MGBFS_HIP(fill_u64(control_.as<uint64_t>(), 0ull, 1, stream_));
-    // The expansion overwrites next_, which holds the previous layer: keep its
-    // first kStashRows rows, which a sample taken after a failed advance reads.
-    stash_count_ = std::min<uint64_t>(prev_count_, kStashRows);
+    // The layer counter keeps running: this layer's children are counted
+    // from counter_base_, its value at the start of the layer.
+    // Chunk the frontier only when the next buffer cannot hold the worst case.
+    const uint64_t chunk = want <= next_cap_ ? count_ : std::max<uint64_t>(1, next_cap_ / gens_.count);
+    // A chunked expansion can stop part way with next_ (which holds the
+    // previous layer) partly overwritten: keep its first kStashRows rows,
+    // which a sample taken after the failed advance reads.  An unchunked one
+    // cannot run out of room.
+    stash_count_ = chunk != count_ ? std::min<uint64_t>(prev_count_, kStashRows) : 0;
     if (stash_count_)
       MGBFS_HIP(hipMemcpyAsync(stash_.as<uint64_t>(), next_.as<uint64_t>(), stash_count_ * 8,
                                hipMemcpyDeviceToDevice, stream_));
-    // Chunk the frontier only when the next buffer cannot hold the worst case.
-    const uint64_t chunk = want <= next_cap_ ? count_ : std::max<uint64_t>(1, next_cap_ / gens_.count);
     MGBFS_HIP(hipEventRecord(ev0_, stream_));
     uint64_t produced = 0;
     for (uint64_t begin = 0; begin < count_; begin += chunk) {
@@ -180,7 +184,7 @@ class BitmapEngine final : public Engine {
       if (chunk != count_) {
         MGBFS_HIP(hipStreamSynchronize(stream_));
         read_control();
-        produced = host_ctl_[0];
+        produced = host_ctl_[0] - counter_base_;
         if (produced + c * gens_.count > next_cap_) {
           *fatal = kErrFutureCap;
           terminal = true;
@@ -190,7 +194,7 @@ class BitmapEngine final : public Engine {
       ++launches;
       MGBFS_HIP(bitmap_expand(front_.as<uint64_t>() + begin, c, gens_, bitmap_.as<uint32_t>(), domain_,
                               next_.as<uint64_t>(), reinterpret_cast<unsigned long long*>(control_.as<uint64_t>()),
-                              next_cap_, error_word(), stream_));
+                              counter_base_, next_cap_, error_word(), stream_));
     }
     MGBFS_HIP(hipEventRecord(ev1_, stream_));
     MGBFS_HIP(hipStreamSynchronize(stream_));
@@ -206,7 +210,8 @@ class BitmapEngine final : public Engine {
       terminal = true;
       return MGBFS_RESOURCE;
     }
-    const uint64_t produced_all = host_ctl_[0];
+    const uint64_t produced_all = host_ctl_[0] - counter_base_;
+    counter_base_ = host_ctl_[0];
     *next_count = produced_all;
     if (produced_all == 0) {
       terminal = true;
@@ -313,6 +318,7 @@ class BitmapEngine final : public Engine {
   DeviceBuffer bitmap_, front_, next_, control_, stash_;
   uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, stash_count_ = 0, start_key_ = 0,
            key_mask_ = ~0ull;
+  uint64_t counter_base_ = 0;  // value of the running child counter at the start of the layer
   bool complete_ = false;
   uint64_t* host_ctl_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -815,6 +821,6 @@ extern "C" int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, 
     }
   mgbfs::fill_shift_groups(pg);
   return mgbfs::bitmap_expand(frontier, count, pg, bitmap, bitmap_bits, next,
-                              reinterpret_cast<unsigned long long*>(next_count), next_capacity, error,
+                              reinterpret_cast<unsigned long long*>(next_count), 0, next_capacity, error,
                               static_cast<hipStream_t>(stream));
 }

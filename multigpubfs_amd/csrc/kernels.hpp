@@ -8,9 +8,11 @@ namespace mgbfs {
 
 hipError_t bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits,
                        uint64_t* frontier, uint32_t* error, hipStream_t stream);
+// New children go to next[*next_count - count_base + ...]: the counter may
+// keep running across layers, count_base being its value at the layer start.
 hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens, uint32_t* bitmap,
-                         uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t next_cap,
-                         uint32_t* error, hipStream_t stream);
+                         uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t count_base,
+                         uint64_t next_cap, uint32_t* error, hipStream_t stream);
 // Sorted-bank backend limits.
 constexpr uint32_t kMaxPassBits = 10;     // buckets per routing pass (LDS counters)
 constexpr uint32_t kMaxBucketBits = 20;   // sub-buckets per layer: 2^20
