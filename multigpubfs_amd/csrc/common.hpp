@@ -116,6 +116,17 @@ __device__ __forceinline__ uint64_t apply_packed(uint64_t key, const uint8_t* sr
   return (static_cast<uint64_t>(chi) << 32) | clo;
 }
 
+// (a & m) | c on both 32-bit halves, one v_and_or_b32 each (the compiler
+// emits a separate AND and OR for this pattern); m is uniform.
+__device__ __forceinline__ uint64_t and_or64(uint64_t a, uint64_t m, uint64_t c) {
+  uint32_t lo, hi;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(lo) : "v"(static_cast<uint32_t>(a)), "s"(static_cast<uint32_t>(m)),
+      "v"(static_cast<uint32_t>(c)));
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(hi) : "v"(static_cast<uint32_t>(a >> 32)),
+      "s"(static_cast<uint32_t>(m >> 32)), "v"(static_cast<uint32_t>(c >> 32)));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Same result as apply_packed, from the shift groups of generator g, for N
 // keys at once: each group's shift and mask are uniform (scalar loads) and
 // are applied to all N keys before the next group is read.
@@ -130,13 +141,13 @@ __device__ __forceinline__ void apply_groups_n(const uint64_t (&key)[N], uint64_
     const uint32_t sh = group_shift(pg, g, j);
     const uint64_t mask = pg.gmask[g][j];
 #pragma unroll
-    for (int m = 0; m < N; ++m) child[m] |= (key[m] >> sh) & mask;
+    for (int m = 0; m < N; ++m) child[m] = and_or64(key[m] >> sh, mask, child[m]);
   }
   for (; j < groups; ++j) {
     const uint32_t sh = group_shift(pg, g, j);
     const uint64_t mask = pg.gmask[g][j];
 #pragma unroll
-    for (int m = 0; m < N; ++m) child[m] |= (key[m] << sh) & mask;
+    for (int m = 0; m < N; ++m) child[m] = and_or64(key[m] << sh, mask, child[m]);
   }
 }
 

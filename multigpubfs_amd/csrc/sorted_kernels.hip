@@ -51,41 +51,19 @@ __device__ __forceinline__ uint32_t range_index(uint64_t h, uint32_t n) {
   return __umulhi(static_cast<uint32_t>(h >> 32), n);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave: DPP row shifts by 1, 2,
+// 4 and 8 (lanes shifted in from outside the 16-lane row read 0) give each
+// row's prefix; row broadcasts of lane 15 into rows 1 and 3, then of lane
+// 31 into rows 2 and 3, carry the row totals.  No lane predicates, no LDS.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t t = __shfl_up(v, off, kWave);
-    if (lane >= static_cast<uint32_t>(off)) v += t;
-  }
-  return v;
-}
-
-// Exclusive scan of `count` (<= 4 * blockDim) LDS counters into `out`.
-__device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* wave_sums) {
-  const uint32_t per = (count + blockDim.x - 1) / blockDim.x;  // <= 4
-  const uint32_t base = threadIdx.x * per;
-  uint32_t v[4];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t i = base + k;
-    v[k] = (k < static_cast<int>(per) && i < count) ? in[i] : 0u;
-    s += v[k];
-  }
-  const uint32_t incl = wave_incl_scan(s);
-  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  if (lane == kWave - 1) wave_sums[wave] = incl;
-  __syncthreads();
-  uint32_t run = incl - s;
-  for (uint32_t w = 0; w < wave; ++w) run += wave_sums[w];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t i = base + k;
-    if (k < static_cast<int>(per) && i < count) out[i] = run;
-    run += v[k];
-  }
-  __syncthreads();
+  int x = static_cast<int>(v);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return static_cast<uint32_t>(x);
 }
 
 // ------------------------------------------------------------ expansion
@@ -100,12 +78,47 @@ __device__ void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t c
 constexpr uint32_t kDropRun = ~0u;
 
 struct TileLds {
-  uint64_t s[kTile];
-  uint32_t off[1u << kMaxPassBits];
-  uint32_t lhist[1u << kMaxPassBits];
-  uint32_t lstart[1u << kMaxPassBits];
-  uint32_t wave_sums[kRouteThreads / kWave];
+  uint64_t s[kTile + kWave];  // + one parking slot per lane for invalid elements
+  alignas(8) uint32_t off[1u << kMaxPassBits];
+  alignas(8) uint32_t lhist[1u << kMaxPassBits];
+  alignas(8) uint32_t lstart[1u << kMaxPassBits];
+  uint32_t spare[kWave];  // rank counters of invalid elements (values unused)
+  alignas(16) uint32_t wave_sums[kRouteThreads / kWave];
 };
+static_assert((1u << kMaxPassBits) == 2 * kRouteThreads, "each routing thread owns two bucket counters");
+
+// Clears every bucket counter (two per thread); entries at or above a
+// pass's bucket count stay zero, so the tile scan runs over all of them.
+__device__ __forceinline__ void clear_tile_counters(TileLds& t) {
+  reinterpret_cast<uint2*>(t.lhist)[threadIdx.x] = make_uint2(0u, 0u);
+}
+
+// Exclusive scan of all bucket counters into t.lstart; thread t owns
+// buckets 2t and 2t+1, returns their counts in c and clears them (no other
+// thread reads them: the next tile's increments come after later barriers).
+// Returns the tile's total; start receives the two buckets' starts.
+__device__ __forceinline__ uint32_t tile_scan(TileLds& t, uint2& c, uint2& start) {
+  c = reinterpret_cast<const uint2*>(t.lhist)[threadIdx.x];
+  reinterpret_cast<uint2*>(t.lhist)[threadIdx.x] = make_uint2(0u, 0u);
+  const uint32_t s = c.x + c.y;
+  const uint32_t incl = wave_incl_scan(s);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(threadIdx.x)) / kWave;  // uniform
+  if (lane == kWave - 1) t.wave_sums[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kRouteThreads / kWave; ++w) {
+    const uint32_t ws = t.wave_sums[w];
+    before += w < wave ? ws : 0u;
+    total += ws;
+  }
+  const uint32_t run = before + incl - s;
+  start = make_uint2(run, run + c.x);
+  reinterpret_cast<uint2*>(t.lstart)[threadIdx.x] = start;
+  __syncthreads();
+  return total;
+}
 
 // Destination indexing.  When the output array (plus a kTile-entry dump
 // area at index `dump`) has fewer than 2^32 entries, t.off[b] holds bucket
@@ -138,41 +151,49 @@ struct RunCursors {
 };
 
 template <int PER, typename BucketFn, typename BaseFn>
-__device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid, uint32_t buckets,
+__device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid,
                                            BucketFn bkt, const RunCursors& runs, BaseFn slab_base,
                                            uint64_t* __restrict__ out, uint32_t dump) {
+  // No branches per element: an invalid element takes its rank from the
+  // lane's spare counter and is parked in the lane's slot past the tile.
+  const uint32_t lane = threadIdx.x & (kWave - 1);
   uint32_t rank[PER];
 #pragma unroll
-  for (int k = 0; k < PER; ++k) rank[k] = ((valid >> k) & 1u) ? atomicAdd(&t.lhist[bkt(v[k])], 1u) : 0u;
+  for (int k = 0; k < PER; ++k) {
+    uint32_t* c = ((valid >> k) & 1u) ? &t.lhist[bkt(v[k])] : &t.spare[lane];
+    rank[k] = atomicAdd(c, 1u);
+  }
   __syncthreads();
-  lds_exclusive_scan(t.lhist, t.lstart, buckets, t.wave_sums);
-  const uint32_t ne = t.lstart[buckets - 1] + t.lhist[buckets - 1];
+  uint2 n, st;
+  const uint32_t ne = tile_scan(t, n, st);
   // Run reservations first: both atomics go out before the LDS placement
   // and their offsets are checked after it, so the round trips overlap the
   // placement.  (A check right after each atomic branches on its result and
   // waits for it there.)  Buckets without a run issue no atomic: a tile's
   // children cluster into few buckets.
-  static_assert((1u << kMaxPassBits) <= 2 * kRouteThreads, "two buckets per thread");
-  const uint32_t b0 = threadIdx.x, b1 = threadIdx.x + kRouteThreads;
-  const uint32_t n0 = b0 < buckets ? t.lhist[b0] : 0u, n1 = b1 < buckets ? t.lhist[b1] : 0u;
+  const uint32_t b0 = 2 * threadIdx.x, b1 = b0 + 1;
   uint32_t off0 = 0, off1 = 0;
-  if (n0) off0 = atomicAdd(runs.cursor(b0), n0);
-  if (n1) off1 = atomicAdd(runs.cursor(b1), n1);
+  if (n.x) off0 = atomicAdd(runs.cursor(b0), n.x);
+  if (n.y) off1 = atomicAdd(runs.cursor(b1), n.y);
+  // All bucket starts are read first (in bounds for any element) and pinned,
+  // so the reads go out together instead of one guarded read per element.
+  uint32_t pos[PER];
 #pragma unroll
-  for (int k = 0; k < PER; ++k)
-    if ((valid >> k) & 1u) t.s[t.lstart[bkt(v[k])] + rank[k]] = v[k];
+  for (int k = 0; k < PER; ++k) pos[k] = t.lstart[bkt(v[k])];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) asm volatile("" : "+v"(pos[k]));
+#pragma unroll
+  for (int k = 0; k < PER; ++k) t.s[((valid >> k) & 1u) ? pos[k] + rank[k] : kTile + lane] = v[k];
   asm volatile("" : "+v"(off0), "+v"(off1));  // the offsets are used only from here on
-  off0 = runs.settle(off0, n0);
-  off1 = runs.settle(off1, n1);
+  off0 = runs.settle(off0, n.x);
+  off1 = runs.settle(off1, n.y);
   if (dump != kWideOut) {
-    auto dest = [&](uint32_t b, uint32_t off) {
-      return (off == kDropRun ? dump : static_cast<uint32_t>(slab_base(b)) + off) - t.lstart[b];
+    auto dest = [&](uint32_t b, uint32_t off, uint32_t start) {
+      return (off == kDropRun ? dump : static_cast<uint32_t>(slab_base(b)) + off) - start;
     };
-    if (b0 < buckets) t.off[b0] = dest(b0, off0);
-    if (b1 < buckets) t.off[b1] = dest(b1, off1);
+    reinterpret_cast<uint2*>(t.off)[threadIdx.x] = make_uint2(dest(b0, off0, st.x), dest(b1, off1, st.y));
   } else {
-    if (b0 < buckets) t.off[b0] = off0;
-    if (b1 < buckets) t.off[b1] = off1;
+    reinterpret_cast<uint2*>(t.off)[threadIdx.x] = make_uint2(off0, off1);
   }
   __syncthreads();
   if (dump != kWideOut) {
@@ -188,9 +209,8 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
       if (off != kDropRun) out[slab_base(b) + off + (j - t.lstart[b])] = h;
     }
   }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
-  __syncthreads();
+  // No barrier here: the next tile's placement and scan come after its
+  // first barrier, which every thread reaches only after its write-out.
 }
 
 // MASKED: every parent carries a non-empty back-edge mask (all layers after
@@ -220,9 +240,8 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
   using T = ExpandTile<G, MASKED>;
   static_assert(T::kRegs <= 32, "valid mask is 32 bits");
   __shared__ TileLds t;
-  const uint32_t buckets = nsub >> K2;  // lanes (sub-buckets when K2 = 0)
   const uint32_t part = blockIdx.x % P;
-  for (uint32_t i = threadIdx.x; i < buckets; i += kRouteThreads) t.lhist[i] = 0;
+  clear_tile_counters(t);
   __syncthreads();
   const RunCursors runs{cursors + part, P, cap, error, err_bit};
   auto slab_base = [=](uint32_t b) { return (static_cast<uint64_t>(b) * P + part) * stride; };
@@ -297,7 +316,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       }
 #pragma unroll
       for (int k = 0; k < 2 * PP; ++k) w[k] = (keymix(w[k], L.km) << L.tagbits) | wt[k];
-      route_regs(t, w, wvalid, buckets, bkt, runs, slab_base, out, dump);
+      route_regs(t, w, wvalid, bkt, runs, slab_base, out, dump);
     } else {
       uint64_t v[T::kRegs];
 #pragma unroll
@@ -308,7 +327,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 #pragma unroll
         for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
       }
-      route_regs(t, v, valid, buckets, bkt, runs, slab_base, out, dump);
+      route_regs(t, v, valid, bkt, runs, slab_base, out, dump);
     }
   }
 }
@@ -327,8 +346,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
                                                                            uint64_t* __restrict__ out,
                                                                            uint32_t dump) {
   __shared__ TileLds t;
-  const uint32_t buckets = 1u << K2;
-  for (uint32_t b = threadIdx.x; b < buckets; b += kRouteThreads) t.lhist[b] = 0;
+  clear_tile_counters(t);
   __syncthreads();
   const uint64_t slab = blockIdx.x;
   const uint64_t r0 = slab * lcap, r1 = r0 + min(lcount[slab], lcap);
@@ -345,7 +363,7 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
       v[k] = i < r1 ? in[i] : 0;
       if (i < r1) valid |= 1u << k;
     }
-    route_regs(t, v, valid, buckets, bkt, runs, slab_base, out, dump);
+    route_regs(t, v, valid, bkt, runs, slab_base, out, dump);
   }
 }
 
