@@ -80,9 +80,10 @@ constexpr uint32_t kDropRun = ~0u;
 struct TileLds {
   uint64_t s[kTile + kWave];  // + one parking slot per lane for invalid elements
   alignas(8) uint32_t off[1u << kMaxPassBits];
-  alignas(8) uint32_t lhist[1u << kMaxPassBits];
+  // Bucket counters, then one spare counter per lane that ranks invalid
+  // elements (its values are never used).
+  alignas(8) uint32_t lhist[(1u << kMaxPassBits) + kWave];
   alignas(8) uint32_t lstart[1u << kMaxPassBits];
-  uint32_t spare[kWave];  // rank counters of invalid elements (values unused)
   alignas(16) uint32_t wave_sums[kRouteThreads / kWave];
 };
 static_assert((1u << kMaxPassBits) == 2 * kRouteThreads, "each routing thread owns two bucket counters");
@@ -160,8 +161,9 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   uint32_t rank[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    uint32_t* c = ((valid >> k) & 1u) ? &t.lhist[bkt(v[k])] : &t.spare[lane];
-    rank[k] = atomicAdd(c, 1u);
+    uint32_t b = bkt(v[k]);
+    asm volatile("" : "+v"(b));  // keeps the choice below a select, not a branch
+    rank[k] = atomicAdd(&t.lhist[((valid >> k) & 1u) ? b : (1u << kMaxPassBits) + lane], 1u);
   }
   __syncthreads();
   uint2 n, st;
@@ -197,9 +199,25 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   }
   __syncthreads();
   if (dump != kWideOut) {
-    for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
-      const uint64_t h = t.s[j];
-      out[t.off[bkt(h)] + j] = h;  // 32-bit index arithmetic, wraps as intended
+    // Rows go out in rounds of kRound per thread: the round's rows are read
+    // at once (rows at or past ne hold stale entries whose bucket index is
+    // still in range), then their run offsets, then the rows below ne are
+    // stored.  A loop over single rows waits on two dependent LDS reads per
+    // row; a single round of all kTilePerThread rows costs the registers of
+    // the second workgroup per CU.
+    constexpr int kRound = kTilePerThread / 2;
+    for (uint32_t r = 0; r < kTile && r < ne; r += kRound * kRouteThreads) {
+      uint64_t h[kRound];
+      uint32_t o[kRound];
+#pragma unroll
+      for (int u = 0; u < kRound; ++u) h[u] = t.s[r + threadIdx.x + u * kRouteThreads];
+#pragma unroll
+      for (int u = 0; u < kRound; ++u) o[u] = t.off[bkt(h[u])];
+#pragma unroll
+      for (int u = 0; u < kRound; ++u) {
+        const uint32_t j = r + threadIdx.x + u * kRouteThreads;
+        if (j < ne) out[o[u] + j] = h[u];  // 32-bit index arithmetic, wraps as intended
+      }
     }
   } else {
     for (uint32_t j = threadIdx.x; j < ne; j += kRouteThreads) {
@@ -211,6 +229,16 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   }
   // No barrier here: the next tile's placement and scan come after its
   // first barrier, which every thread reaches only after its write-out.
+}
+
+// A parent's key for the shift-group expansion: keyunmix without its final
+// mask, so bits above the key's 4n bits are left as the multiply leaves
+// them.  Every shift group keeps only destination nibbles whose source
+// nibble lies below n, so those bits never reach a child.
+__device__ __forceinline__ uint64_t parent_key(uint64_t h, const SortedLayout& L) {
+  uint64_t x = h >> L.tagbits;
+  x ^= x >> L.km.shift;
+  return x * L.km.i1;
 }
 
 // MASKED: every parent carries a non-empty back-edge mask (all layers after
@@ -266,7 +294,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
     uint64_t key[T::kParentsPerThread];
 #pragma unroll
     for (int m = 0; m < T::kParentsPerThread; ++m) {
-      key[m] = keyunmix(pv[m] >> L.tagbits, L.km);
+      key[m] = parent_key(pv[m], L);
       uint32_t want = (1u << G) - 1u;
       if (MASKED) {
         const uint32_t skip = static_cast<uint32_t>(pv[m] & L.tagmask);
