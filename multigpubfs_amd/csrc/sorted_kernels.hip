@@ -551,13 +551,19 @@ __device__ __forceinline__ void load_slab(const UniqueArgs& a, uint32_t f, Bucke
   for (int k = 0; k < kUniquePerThread; ++k) d.v[k] = a.cand[s + k * kUniqueThreads + threadIdx.x];
 }
 
+// Unconditional loads: a slot past its slice reads the sub-bucket's own
+// slab instead (always allocated; the value is never used).  With the loads
+// under branches the number in flight at the insert loop is unknown there,
+// so its first use of a candidate register waited for every bank load too.
 __device__ __forceinline__ void load_bank(const UniqueArgs& a, const BucketBounds& b, BucketData& d) {
   const int ncur = cur_prefetch(a);
+  const uint64_t* spare = a.cand + b.s + threadIdx.x;
 #pragma unroll
   for (int k = 0; k < kBankPrefetch; ++k) {
     uint64_t i;
     const bool live = bank_slot(a, b, k, &i);
-    d.bpre[k] = live ? (k < ncur ? a.cur : a.prev)[i] : 0;
+    const uint64_t* bank = k < ncur ? a.cur : a.prev;
+    d.bpre[k] = *(live ? bank + i : spare);
   }
 }
 
@@ -729,10 +735,10 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         }
       }
     }
-    // One LDS atomic per wave for the occupied-slot count.
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) occupied += __shfl_xor(occupied, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && occupied) atomicAdd(&u.unique, occupied);
+    // One LDS atomic per wave for the occupied-slot count (the last lane of
+    // the wave's prefix sum holds the wave's total).
+    occupied = wave_incl_scan(occupied);
+    if ((threadIdx.x & (kWave - 1)) == kWave - 1 && occupied) atomicAdd(&u.unique, occupied);
     __syncthreads();
     // One wave turns the fine counts into the fine buckets' first rows
     // (exclusive scan by shuffles, <= 16 counters per lane) while the
