@@ -385,10 +385,12 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
   for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
     uint64_t v[kTilePerThread];
     uint32_t valid = 0;
+    // Loads past the slab's count re-read its last entry (t0 < r1), so none
+    // is under a branch and each rank atomic waits only for its own load.
 #pragma unroll
     for (int k = 0; k < kTilePerThread; ++k) {
       const uint64_t i = t0 + static_cast<uint64_t>(k) * kRouteThreads + threadIdx.x;
-      v[k] = i < r1 ? in[i] : 0;
+      v[k] = in[i < r1 ? i : r1 - 1];
       if (i < r1) valid |= 1u << k;
     }
     route_regs(t, v, valid, bkt, runs, slab_base, out, dump);
