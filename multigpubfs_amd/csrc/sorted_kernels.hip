@@ -791,26 +791,7 @@ __global__ __launch_bounds__(kRouteThreads) void compact_kernel(const uint64_t* 
   const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
   const uint32_t sh = fb;
   const uint64_t lo = off[static_cast<uint64_t>(f) << sh], hi = off[static_cast<uint64_t>(f + 1) << sh];
-  const uint64_t n = hi - lo;
-  // A thread's rows go out in rounds of kCopyRound: all loads first (rows
-  // past n re-read the last survivor, so none is under a branch), then the
-  // stores, instead of one dependent load and store per loop trip.
-  constexpr int kCopyRound = 8;
-  for (uint64_t base = threadIdx.x; base < n; base += kCopyRound * kRouteThreads) {
-    uint64_t v[kCopyRound];
-#pragma unroll
-    for (int u = 0; u < kCopyRound; ++u) {
-      const uint64_t i = base + static_cast<uint64_t>(u) * kRouteThreads;
-      v[u] = cand[s + (i < n ? i : n - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < kCopyRound; ++u) asm volatile("" : "+v"(v[u]));  // no store before the last load
-#pragma unroll
-    for (int u = 0; u < kCopyRound; ++u) {
-      const uint64_t i = base + static_cast<uint64_t>(u) * kRouteThreads;
-      if (i < n) bank[lo + i] = v[u];
-    }
-  }
+  for (uint64_t i = threadIdx.x; i < hi - lo; i += kRouteThreads) bank[lo + i] = cand[s + i];
 }
 
 // Start layer: bank[0] = mix(start); offsets over 2^Ko fine buckets.
