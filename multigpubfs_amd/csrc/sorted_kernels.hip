@@ -652,20 +652,16 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
       while (left) {
         const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&u.table[slot]), kEmpty, h);
-        bool done = old == kEmpty;
-        if (!done && (old | a.tagmask) == (h | a.tagmask)) {
-          if (a.merge) atomicOr(reinterpret_cast<unsigned long long*>(&u.table[slot]), h & a.tagmask);
-          done = true;
-        }
-        if (done) {
-          --left;
+        const bool dup = old != kEmpty && (old | a.tagmask) == (h | a.tagmask);
+        if (dup && a.merge) atomicOr(reinterpret_cast<unsigned long long*>(&u.table[slot]), h & a.tagmask);
+        // The list advance as selects: a branch per round costs more scalar
+        // exec-mask work than the selects cost vector work.
+        const bool done = old == kEmpty || dup;
+        left -= done ? 1u : 0u;
 #pragma unroll
-          for (int k = 0; k + 1 < kUniquePerThread; ++k) dcur.v[k] = dcur.v[k + 1];
-          h = dcur.v[0];
-          slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
-        } else {
-          slot = (slot + 1) & mask;
-        }
+        for (int k = 0; k + 1 < kUniquePerThread; ++k) dcur.v[k] = done ? dcur.v[k + 1] : dcur.v[k];
+        h = dcur.v[0];
+        slot = done ? static_cast<uint32_t>(h >> a.tagbits) & mask : (slot + 1) & mask;
       }
     }
     __syncthreads();
@@ -688,21 +684,15 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
       while (left) {
         const uint64_t v = (live & 1u) ? u.table[slot] : kEmpty;
-        bool done = v == kEmpty;
-        if (!done && (v | a.tagmask) == (h | a.tagmask)) {
-          u.table[slot] = v | a.mark;
-          done = true;
-        }
-        if (done) {
-          --left;
-          live >>= 1;
+        const bool match = v != kEmpty && (v | a.tagmask) == (h | a.tagmask);
+        if (match) u.table[slot] = v | a.mark;
+        const bool done = v == kEmpty || match;  // the list advance as selects, as in the inserts
+        left -= done ? 1u : 0u;
+        live = done ? live >> 1 : live;
 #pragma unroll
-          for (int k = 0; k + 1 < kBankPrefetch; ++k) dcur.bpre[k] = dcur.bpre[k + 1];
-          h = dcur.bpre[0];
-          slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
-        } else {
-          slot = (slot + 1) & mask;
-        }
+        for (int k = 0; k + 1 < kBankPrefetch; ++k) dcur.bpre[k] = done ? dcur.bpre[k + 1] : dcur.bpre[k];
+        h = dcur.bpre[0];
+        slot = done ? static_cast<uint32_t>(h >> a.tagbits) & mask : (slot + 1) & mask;
       }
     }
     const int ncur = cur_prefetch(a);
