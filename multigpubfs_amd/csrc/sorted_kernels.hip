@@ -151,10 +151,13 @@ struct RunCursors {
   }
 };
 
-template <int PER, typename BucketFn, typename BaseFn>
+// `between` runs after the tile is placed in LDS and before its write-out:
+// the caller issues the next tile's loads there (v is no longer read), so
+// they are in flight during the write-out.
+template <int PER, typename BucketFn, typename BaseFn, typename BetweenFn>
 __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER], uint32_t valid,
                                            BucketFn bkt, const RunCursors& runs, BaseFn slab_base,
-                                           uint64_t* __restrict__ out, uint32_t dump) {
+                                           uint64_t* __restrict__ out, uint32_t dump, BetweenFn between) {
   // No branches per element: an invalid element takes its rank from the
   // lane's spare counter and is parked in the lane's slot past the tile.
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -198,6 +201,7 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
     reinterpret_cast<uint2*>(t.off)[threadIdx.x] = make_uint2(off0, off1);
   }
   __syncthreads();
+  between();
   if (dump != kWideOut) {
     // Rows go out in rounds of kRound per thread: the round's rows are read
     // at once (rows at or past ne hold stale entries whose bucket index is
@@ -277,6 +281,10 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
   // Tiles of contiguous parents go to blocks round-robin (tile i to block
   // i mod NB), so every lane part samples the whole frontier.
   const uint64_t tiles = (F + T::kParents - 1) / T::kParents;
+  // Parents are loaded at the top of their own tile: loading the next
+  // tile's during the write-out (as the partition does) takes
+  // expand_scatter past 128 VGPRs, one workgroup per CU.
+  auto next = [] {};
   for (uint64_t tile = blockIdx.x; tile < tiles; tile += NB) {
     const uint64_t t0 = tile * T::kParents;
     // Loads past the frontier re-read its last parent (F >= 1), so every
@@ -344,7 +352,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
       }
 #pragma unroll
       for (int k = 0; k < 2 * PP; ++k) w[k] = (keymix(w[k], L.km) << L.tagbits) | wt[k];
-      route_regs(t, w, wvalid, bkt, runs, slab_base, out, dump);
+      route_regs(t, w, wvalid, bkt, runs, slab_base, out, dump, next);
     } else {
       uint64_t v[T::kRegs];
 #pragma unroll
@@ -355,7 +363,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
 #pragma unroll
         for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
       }
-      route_regs(t, v, valid, bkt, runs, slab_base, out, dump);
+      route_regs(t, v, valid, bkt, runs, slab_base, out, dump, next);
     }
   }
 }
@@ -382,18 +390,26 @@ __global__ __launch_bounds__(kRouteThreads) void partition_scatter_kernel(const 
   const uint64_t first = static_cast<uint64_t>(blockIdx.x / P) << K2;  // first sub-bucket of this lane
   const RunCursors runs{scount + first, 1, kUniqueCap, error, kErrBucketCap};
   auto slab_base = [=](uint32_t b) { return (first + b) * kSlabStride; };
-  for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
-    uint64_t v[kTilePerThread];
-    uint32_t valid = 0;
-    // Loads past the slab's count re-read its last entry (t0 < r1), so none
-    // is under a branch and each rank atomic waits only for its own load.
+  // Loads past the slab's count re-read its last entry (t0 < r1), so none
+  // is under a branch and each rank atomic waits only for its own load.  A
+  // tile's entries are loaded during the previous tile's write-out.
+  uint64_t v[kTilePerThread];
+  uint32_t valid = 0;
+  auto load_tile = [&](uint64_t t0) {
+    valid = 0;
 #pragma unroll
     for (int k = 0; k < kTilePerThread; ++k) {
       const uint64_t i = t0 + static_cast<uint64_t>(k) * kRouteThreads + threadIdx.x;
       v[k] = in[i < r1 ? i : r1 - 1];
       if (i < r1) valid |= 1u << k;
     }
-    route_regs(t, v, valid, bkt, runs, slab_base, out, dump);
+  };
+  if (r0 < r1) load_tile(r0);
+  for (uint64_t t0 = r0; t0 < r1; t0 += kTile) {
+    auto next = [&] {
+      if (t0 + kTile < r1) load_tile(t0 + kTile);
+    };
+    route_regs(t, v, valid, bkt, runs, slab_base, out, dump, next);
   }
 }
 
