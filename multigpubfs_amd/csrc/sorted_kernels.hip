@@ -590,6 +590,16 @@ __device__ __forceinline__ uint32_t filter_bit(uint64_t h, uint32_t tagbits) {
   return static_cast<uint32_t>((h >> tagbits) >> 13) & (kFilterWords * 32 - 1);
 }
 
+// Entry i & (kBankPrefetch - 1) of a lane's prefetched bank entries, by a
+// select tree on the index bits (register arrays cannot be indexed per lane).
+__device__ __forceinline__ uint64_t pick_entry(const uint64_t (&e)[kBankPrefetch], uint32_t i) {
+  static_assert(kBankPrefetch == 8, "three index bits");
+  const bool b0 = i & 1u, b1 = i & 2u, b2 = i & 4u;
+  const uint64_t a0 = b0 ? e[1] : e[0], a1 = b0 ? e[3] : e[2], a2 = b0 ? e[5] : e[4], a3 = b0 ? e[7] : e[6];
+  const uint64_t c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2;
+  return b2 ? c1 : c0;
+}
+
 constexpr int kPairsPerThread = kUniqueSlots / 2 / kUniqueThreads;  // table slot pairs per thread
 
 // Block x handles sub-buckets x, x + grid, ... (the engine launches one
@@ -681,10 +691,12 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
       }
     }
     __syncthreads();
-    // Difference against the retained layers (same sub-bucket only), the
-    // prefetched entries as one list per lane like the inserts; list
-    // positions past a bank's slice, or whose filter bit is clear (no
-    // candidate has that state), are skipped without a table read.
+    // Difference against the retained layers (same sub-bucket only).  Of
+    // the prefetched entries only the live ones are probed: inside a bank's
+    // slice and with their filter bit set (some candidate may have that
+    // state).  Each round probes the lane's lowest live entry, picked by its
+    // index, so a lane's rounds follow its live entries and their probe
+    // lengths, not its list length.
     {
       uint32_t live = 0;
 #pragma unroll
@@ -695,19 +707,15 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
           if ((u.filter[b >> 5] >> (b & 31)) & 1u) live |= 1u << k;
         }
       }
-      uint32_t left = live ? 32 - __builtin_clz(live) : 0;
-      uint64_t h = dcur.bpre[0];
+      uint64_t h = pick_entry(dcur.bpre, __builtin_ctz(live | (1u << kBankPrefetch)));
       uint32_t slot = static_cast<uint32_t>(h >> a.tagbits) & mask;
-      while (left) {
-        const uint64_t v = (live & 1u) ? u.table[slot] : kEmpty;
+      while (live) {
+        const uint64_t v = u.table[slot];
         const bool match = v != kEmpty && (v | a.tagmask) == (h | a.tagmask);
         if (match) u.table[slot] = v | a.mark;
-        const bool done = v == kEmpty || match;  // the list advance as selects, as in the inserts
-        left -= done ? 1u : 0u;
-        live = done ? live >> 1 : live;
-#pragma unroll
-        for (int k = 0; k + 1 < kBankPrefetch; ++k) dcur.bpre[k] = done ? dcur.bpre[k + 1] : dcur.bpre[k];
-        h = dcur.bpre[0];
+        const bool done = v == kEmpty || match;
+        live = done ? live & (live - 1) : live;
+        h = done ? pick_entry(dcur.bpre, __builtin_ctz(live | (1u << kBankPrefetch))) : h;
         slot = done ? static_cast<uint32_t>(h >> a.tagbits) & mask : (slot + 1) & mask;
       }
     }
