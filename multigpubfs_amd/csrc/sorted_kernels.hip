@@ -420,7 +420,8 @@ constexpr uint32_t kFilterWords = 2048;  // 64 Ki bits
 struct UniqueLds {
   alignas(16) uint64_t table[kUniqueSlots];  // read two slots at a time
   alignas(16) uint32_t filter[kFilterWords];  // one bit per candidate state (key bits 13..28)
-  uint32_t fhist[1u << kMaxFineBits];        // survivors, then write cursors, per fine bucket
+  uint32_t fhist[(1u << kMaxFineBits) + kWave];  // survivors, then write cursors, per fine bucket;
+                                                  // then one spare counter per lane
   uint32_t unique;
 };
 
@@ -725,31 +726,31 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // Survivors are the occupied, unmarked slots; each entry carries the
     // merged back-edge mask of its duplicates.  Thread t owns slot pairs
-    // t + j * kUniqueThreads (conflict-free 16-byte reads).  The count
-    // atomic's return value is the survivor's rank within its fine bucket,
-    // kept (16 bits each, two per pair) for the write-out.
+    // t + j * kUniqueThreads (conflict-free 16-byte reads).  All its pairs
+    // are read first (pairs past a smaller table hold stale entries and are
+    // masked out), then every slot's counting atomic goes out: a survivor's
+    // on its fine bucket's counter, whose return value is its rank there
+    // (kept, 16 bits each, two per pair, for the write-out), any other
+    // slot's on the lane's spare counter.  One guarded read-then-atomics
+    // step per pair waited on three dependent LDS round trips per pair.
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t spare = (1u << kMaxFineBits) + lane;
     uint32_t survive = 0, occupied = 0;
     uint32_t rank[kPairsPerThread];
+    ulonglong2 pairs[kPairsPerThread];
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) pairs[j] = table2[threadIdx.x + j * kUniqueThreads];
 #pragma unroll
     for (int j = 0; j < kPairsPerThread; ++j) {
-      rank[j] = 0;
-      const uint32_t i = threadIdx.x + j * kUniqueThreads;
-      if (2 * i >= slots) continue;
-      const ulonglong2 p = table2[i];
-      if (p.x != kEmpty) {
-        ++occupied;
-        if (!(p.x & a.mark)) {
-          survive |= 1u << (2 * j);
-          rank[j] = atomicAdd(&u.fhist[fine_of(p.x, f)], 1u);
-        }
-      }
-      if (p.y != kEmpty) {
-        ++occupied;
-        if (!(p.y & a.mark)) {
-          survive |= 2u << (2 * j);
-          rank[j] |= atomicAdd(&u.fhist[fine_of(p.y, f)], 1u) << 16;
-        }
-      }
+      const bool in = 2 * (threadIdx.x + j * kUniqueThreads) < slots;
+      const ulonglong2 p = pairs[j];
+      const bool ox = in && p.x != kEmpty, oy = in && p.y != kEmpty;
+      const bool sx = ox && !(p.x & a.mark), sy = oy && !(p.y & a.mark);
+      occupied += (ox ? 1u : 0u) + (oy ? 1u : 0u);
+      survive |= ((sx ? 1u : 0u) | (sy ? 2u : 0u)) << (2 * j);
+      const uint32_t rx = atomicAdd(&u.fhist[sx ? fine_of(p.x, f) : spare], 1u);
+      const uint32_t ry = atomicAdd(&u.fhist[sy ? fine_of(p.y, f) : spare], 1u);
+      rank[j] = (rx & 0xffffu) | (ry << 16);
     }
     // One LDS atomic per wave for the occupied-slot count (the last lane of
     // the wave's prefix sum holds the wave's total).
@@ -774,14 +775,19 @@ __global__ __launch_bounds__(kUniqueThreads) __attribute__((amdgpu_waves_per_eu(
         }
     }
     __syncthreads();
-    // Survivors are re-read from the table and written back in place at
-    // their fine bucket's first row plus their rank.
+    // Survivors are written back in place at their fine bucket's first row
+    // plus their rank: every slot's first-row read goes out first (a stale
+    // or empty slot's fine index is clamped into range), then the stores.
+    uint32_t first[2 * kPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kPairsPerThread; ++j) {
-      if (!((survive >> (2 * j)) & 3u)) continue;
-      const ulonglong2 p = table2[threadIdx.x + j * kUniqueThreads];
-      if ((survive >> (2 * j)) & 1u) a.cand[s + u.fhist[fine_of(p.x, f)] + (rank[j] & 0xffffu)] = p.x;
-      if ((survive >> (2 * j)) & 2u) a.cand[s + u.fhist[fine_of(p.y, f)] + (rank[j] >> 16)] = p.y;
+      first[2 * j] = u.fhist[fine_of(pairs[j].x, f)];
+      first[2 * j + 1] = u.fhist[fine_of(pairs[j].y, f)];
+    }
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+      if ((survive >> (2 * j)) & 1u) a.cand[s + first[2 * j] + (rank[j] & 0xffffu)] = pairs[j].x;
+      if ((survive >> (2 * j)) & 2u) a.cand[s + first[2 * j + 1] + (rank[j] >> 16)] = pairs[j].y;
     }
     if (threadIdx.x == 0) {
       unique_block += u.unique;
