@@ -187,8 +187,12 @@ __device__ __forceinline__ void route_regs(TileLds& t, const uint64_t (&v)[PER],
   for (int k = 0; k < PER; ++k) pos[k] = t.lstart[bkt(v[k])];
 #pragma unroll
   for (int k = 0; k < PER; ++k) asm volatile("" : "+v"(pos[k]));
+  // The element predicates are taken afresh from the mask here: reusing the
+  // rank step's keeps 2 x PER SGPRs live across the scan, and they spill.
+  uint32_t vmask = valid;
+  asm volatile("" : "+v"(vmask));
 #pragma unroll
-  for (int k = 0; k < PER; ++k) t.s[((valid >> k) & 1u) ? pos[k] + rank[k] : kTile + lane] = v[k];
+  for (int k = 0; k < PER; ++k) t.s[((vmask >> k) & 1u) ? pos[k] + rank[k] : kTile + lane] = v[k];
   asm volatile("" : "+v"(off0), "+v"(off1));  // the offsets are used only from here on
   off0 = runs.settle(off0, n.x);
   off1 = runs.settle(off1, n.y);
