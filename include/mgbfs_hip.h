@@ -202,8 +202,8 @@ int mgbfs_packed_bitmap_expand(uint32_t degree, uint32_t generators, const uint3
  * three-bank rule of crates/mgbfs-runtime/src/generic_distributed_run.rs:40)
  * and compacted into the next bank.
  *
- * Bank format: one uint64 per state, (mix(key) << (64 - 4*degree)) | tag,
- * where mix is a bijection on 4*degree bits and the tag holds the state's
+ * Bank format: one uint64 per state, (mix(key) << (64 - b)) | tag with
+ * b = max(32, 4*degree), where mix is a bijection on b bits and the tag holds the state's
  * back-edge mask; a bank is grouped by `buckets` fine buckets with an offset
  * table of buckets + 1 uint32 entries (offsets[buckets] = states).  Offset
  * tables must hold MGBFS_SORTED_OFFSETS entries.  The seed bank has

@@ -140,8 +140,13 @@ __device__ __forceinline__ void apply_groups_n(const uint64_t (&key)[N], uint64_
   for (; j < right; ++j) {
     const uint32_t sh = group_shift(pg, g, j);
     const uint64_t mask = pg.gmask[g][j];
+    if (sh == 0) {  // fixed nibbles (a uniform branch: a 64-bit shift takes two issue slots)
 #pragma unroll
-    for (int m = 0; m < N; ++m) child[m] = and_or64(key[m] >> sh, mask, child[m]);
+      for (int m = 0; m < N; ++m) child[m] = and_or64(key[m], mask, child[m]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < N; ++m) child[m] = and_or64(key[m] >> sh, mask, child[m]);
+    }
   }
   for (; j < groups; ++j) {
     const uint32_t sh = group_shift(pg, g, j);

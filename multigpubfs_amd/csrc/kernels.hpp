@@ -21,10 +21,14 @@ constexpr uint32_t kUniqueSlots = 8192;   // LDS hash slots per sub-bucket
 constexpr uint32_t kUniqueCap = 5120;     // candidates per sub-bucket (table load factor <= 0.625)
 constexpr uint32_t kSlabStride = 5120;    // entries between sub-bucket slab starts (>= kUniqueCap)
 
-// Stored-value layout of the sorted backend: (keymix(key) << tagbits) | tag.
+// Stored-value layout of the sorted backend: (keymix(key) << tagbits) | tag,
+// keymix on b = max(32, 4n) bits (at least 32, so keymix's mask leaves the
+// low word whole) and tagbits = 64 - b.
+inline uint32_t sorted_key_bits(uint32_t degree) { return degree >= 8 ? 4 * degree : 32u; }
+
 struct SortedLayout {
   KeyMix km;
-  uint32_t tagbits;  // 64 - 4n
+  uint32_t tagbits;  // 64 - sorted_key_bits(n)
   uint32_t prune;    // back-edge masks in use: tag bit i = skip generator i
   uint64_t tagmask;  // (1 << tagbits) - 1
 };

@@ -62,8 +62,8 @@ bool make_plan(uint32_t degree, uint32_t generators, const uint32_t* permutation
   }
   fill_shift_groups(p.gens);
   if (!fill_back_edge_masks(p.gens)) return false;
-  p.layout.km = make_keymix(4 * degree);
-  p.layout.tagbits = 64 - 4 * degree;
+  p.layout.km = make_keymix(sorted_key_bits(degree));
+  p.layout.tagbits = 64 - sorted_key_bits(degree);
   p.layout.tagmask = (1ull << p.layout.tagbits) - 1;
   p.layout.prune = generators + 2 <= p.layout.tagbits ? 1u : 0u;
   if (!p.layout.prune)
@@ -200,8 +200,8 @@ extern "C" int mgbfs_sorted_seed(uint32_t degree, uint64_t start_key, uint64_t* 
                                  void* stream) {
   if (degree == 0 || degree > 15 || !bank || !offsets || (start_key >> (4 * degree)) != 0) return hipErrorInvalidValue;
   SortedLayout L{};
-  L.km = make_keymix(4 * degree);
-  L.tagbits = 64 - 4 * degree;
+  L.km = make_keymix(sorted_key_bits(degree));
+  L.tagbits = 64 - sorted_key_bits(degree);
   L.tagmask = (1ull << L.tagbits) - 1;
   return sorted_seed(start_key, L, bank, offsets, kMaxBucketBits, static_cast<hipStream_t>(stream));
 }
@@ -227,8 +227,8 @@ extern "C" int mgbfs_sorted_layer_noalloc(const mgbfs_sorted_layer_args* args, v
 
 extern "C" int mgbfs_sorted_unpack(uint32_t degree, const uint64_t* values, uint64_t count, uint64_t* keys) {
   if (degree == 0 || degree > 15 || (count && (!values || !keys))) return hipErrorInvalidValue;
-  const KeyMix km = make_keymix(4 * degree);
-  const uint32_t tagbits = 64 - 4 * degree;
+  const KeyMix km = make_keymix(sorted_key_bits(degree));
+  const uint32_t tagbits = 64 - sorted_key_bits(degree);
   for (uint64_t i = 0; i < count; ++i) keys[i] = keyunmix(values[i] >> tagbits, km);
   return 0;
 }

@@ -43,8 +43,8 @@ class SortedEngine final : public Engine {
     cap_limit_ = o.capacity ? o.capacity : ~0ull;
     flags_ = o.flags;
     // Stored-value layout and parent-edge tags.
-    layout_.km = make_keymix(4 * n_);
-    layout_.tagbits = 64 - 4 * n_;
+    layout_.km = make_keymix(sorted_key_bits(n_));
+    layout_.tagbits = 64 - sorted_key_bits(n_);
     layout_.tagmask = layout_.tagbits ? ((1ull << layout_.tagbits) - 1) : 0;
     set_generators(g);
     // One workgroup per sub-bucket: persistent grids of 2 or 8 workgroups

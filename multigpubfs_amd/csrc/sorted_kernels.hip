@@ -249,6 +249,16 @@ __device__ __forceinline__ uint64_t parent_key(uint64_t h, const SortedLayout& L
   return x * L.km.i1;
 }
 
+// The stored value (keymix(child) << tagbits) | tag.  The layout's key
+// width is at least 32 bits, so keymix's mask is applied to the high word
+// only (its low word is all ones).
+__device__ __forceinline__ uint64_t stored_value(uint64_t child, uint64_t tag, const SortedLayout& L) {
+  uint64_t x = child * L.km.m1;
+  x &= (static_cast<uint64_t>(static_cast<uint32_t>(L.km.mask >> 32)) << 32) | 0xffffffffull;
+  x ^= x >> L.km.shift;
+  return (x << L.tagbits) | tag;
+}
+
 // MASKED: every parent carries a non-empty back-edge mask (all layers after
 // the start when back-edge pruning is on) and yields at most G-1 children,
 // so kTilePerThread / (G-1) parents fit a tile; each parent keeps G child
@@ -355,7 +365,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
         wvalid |= ((1u << __popc(ok)) - 1u) << (2 * m);  // <= 2 children (non-empty mask)
       }
 #pragma unroll
-      for (int k = 0; k < 2 * PP; ++k) w[k] = (keymix(w[k], L.km) << L.tagbits) | wt[k];
+      for (int k = 0; k < 2 * PP; ++k) w[k] = stored_value(w[k], wt[k], L);
       route_regs(t, w, wvalid, bkt, runs, slab_base, out, dump, next);
     } else {
       uint64_t v[T::kRegs];
@@ -365,7 +375,7 @@ __global__ __launch_bounds__(kRouteThreads) void expand_scatter_kernel(const uin
         apply_groups_n(key, child, gens, g);
         const uint64_t tag = gens.child_mask[g];
 #pragma unroll
-        for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = (keymix(child[m], L.km) << L.tagbits) | tag;
+        for (int m = 0; m < T::kParentsPerThread; ++m) v[m * G + g] = stored_value(child[m], tag, L);
       }
       route_regs(t, v, valid, bkt, runs, slab_base, out, dump, next);
     }
