@@ -249,6 +249,24 @@ __host__ __device__ __forceinline__ uint64_t unmix64(uint64_t x) {
 // layers evenly; callers must not rely on keys that differ only in a few
 // low nibbles landing far apart (the sorted backend samples its routing
 // tiles across the whole frontier for that reason).
+// Ranges of the top 32 bits: sub-bucket f of n holds x with
+// floor(x n / 2^32) = f, i.e. x >= ceil(f 2^32 / n).  range_start computes
+// that bound for 0 <= f < n <= 2^31 without a 64-bit division (dozens of
+// scalar instructions on the GPU), from inv = range_inverse(n) =
+// floor((2^64 - 1) / n): the product estimate floor(f inv / 2^32) is
+// floor(f 2^32 / n) or one less, one comparison fixes it, a second rounds
+// it up.
+__host__ __device__ __forceinline__ uint64_t range_inverse(uint32_t n) { return ~0ull / n; }
+
+__host__ __device__ __forceinline__ uint32_t range_start(uint32_t f, uint32_t n, uint64_t inv) {
+  const uint64_t whole = static_cast<uint64_t>(f) << 32;
+  const uint64_t lo = static_cast<uint64_t>(f) * static_cast<uint32_t>(inv);
+  uint64_t x = static_cast<uint64_t>(f) * static_cast<uint32_t>(inv >> 32) + (lo >> 32);
+  if ((x + 1) * n <= whole) ++x;
+  if (x * n < whole) ++x;
+  return static_cast<uint32_t>(x);
+}
+
 __host__ __device__ __forceinline__ uint64_t keymix(uint64_t x, const KeyMix& k) {
   x = (x * k.m1) & k.mask;
   x ^= x >> k.shift;

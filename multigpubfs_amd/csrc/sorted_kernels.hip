@@ -454,6 +454,7 @@ struct UniqueArgs {
   uint64_t mark;     // top tag bit, never set in a candidate: "present in a bank"
   uint32_t merge;    // OR the back-edge masks of duplicates into the kept entry
   uint32_t fb, nbo_c, nbo_p, tagbits, buckets;  // buckets: sub-buckets of this layer
+  uint64_t range_inv;                           // range_inverse(buckets)
 };
 
 constexpr uint64_t kEmpty = ~0ull;
@@ -501,10 +502,9 @@ struct BucketBounds {
   uint32_t clo, chi, plo, phi;
 };
 
-
-// load_bounds in two steps: the loads, then the arithmetic on their values,
-// for a sub-bucket f < a.buckets with a current bank (and, when kPrev, a
-// previous one).  Unconditional loads: a load under a branch leaves a
+// A sub-bucket's bounds in two steps: the loads, then the arithmetic on
+// their values, for a sub-bucket f < a.buckets with a current bank (and,
+// when kPrev, a previous one).  Unconditional loads: a load under a branch leaves a
 // register copy at the merge, and the copy waits for the load.
 struct RawBounds {
   uint32_t count, clo, chi, plo, phi;
@@ -512,10 +512,13 @@ struct RawBounds {
 
 template <bool kPrev>
 __device__ __forceinline__ RawBounds load_raw_bounds(const UniqueArgs& a, uint32_t f) {
-  const uint64_t n = a.buckets;
-  const uint32_t xlo = static_cast<uint32_t>(((static_cast<uint64_t>(f) << 32) + n - 1) / n);
+  // Sub-bucket f holds top-32 values [range_start(f), range_start(f + 1));
+  // a bank's slice is every fine bucket meeting that range (a boundary
+  // bucket may add states of a neighbour, which match nothing here).
+  const uint32_t n = a.buckets;
+  const uint32_t xlo = range_start(f, n, a.range_inv);
   const bool last = f + 1 == n;
-  const uint32_t xhi = last ? 0u : static_cast<uint32_t>(((static_cast<uint64_t>(f + 1) << 32) + n - 1) / n) - 1u;
+  const uint32_t xhi = last ? 0u : range_start(f + 1, n, a.range_inv) - 1u;
   RawBounds r{a.scount[f], a.cur_off[__umulhi(xlo, a.nbo_c)],
               a.cur_off[last ? a.nbo_c : __umulhi(xhi, a.nbo_c) + 1], 0, 0};
   if constexpr (kPrev) {
@@ -528,29 +531,6 @@ __device__ __forceinline__ RawBounds load_raw_bounds(const UniqueArgs& a, uint32
 __device__ __forceinline__ BucketBounds finish_bounds(uint32_t f, const RawBounds& r) {
   const uint64_t s = static_cast<uint64_t>(f) * kSlabStride;
   return BucketBounds{s, s + r.count, r.clo, r.chi, r.plo, r.phi};
-}
-
-__device__ __forceinline__ BucketBounds load_bounds(const UniqueArgs& a, uint32_t f) {
-  BucketBounds b{0, 0, 0, 0, 0, 0};
-  if (f >= a.buckets) return b;
-  b.s = static_cast<uint64_t>(f) * kSlabStride;
-  b.e = b.s + a.scount[f];
-  // Sub-bucket f holds top-32 values [ceil(f 2^32 / n), ceil((f+1) 2^32 / n));
-  // a bank's slice is every fine bucket meeting that range (a boundary
-  // bucket may add states of a neighbour, which match nothing here).
-  const uint64_t n = a.buckets;
-  const uint32_t xlo = static_cast<uint32_t>(((static_cast<uint64_t>(f) << 32) + n - 1) / n);
-  const bool last = f + 1 == n;
-  const uint32_t xhi = last ? 0u : static_cast<uint32_t>(((static_cast<uint64_t>(f + 1) << 32) + n - 1) / n) - 1u;
-  if (a.cur) {
-    b.clo = a.cur_off[__umulhi(xlo, a.nbo_c)];
-    b.chi = a.cur_off[last ? a.nbo_c : __umulhi(xhi, a.nbo_c) + 1];
-  }
-  if (a.prev) {
-    b.plo = a.prev_off[__umulhi(xlo, a.nbo_p)];
-    b.phi = a.prev_off[last ? a.nbo_p : __umulhi(xhi, a.nbo_p) + 1];
-  }
-  return b;
 }
 
 // The first bank entries: 2 kPrefetch per thread, all from the current
@@ -928,7 +908,7 @@ hipError_t sorted_unique_diff(uint64_t* cand, const uint32_t* scount, uint32_t n
   // G <= tagbits - 2), so the mark bit tagbits-1 is free and a marked entry
   // never equals the all-ones empty marker.
   UniqueArgs a{cand, scount, cur, cur_off, prev, prev_off, fine_counts, error, unique_total, cand_total, tagmask,
-               1ull << (tagbits - 1), merge ? 1u : 0u, fb, nbo_c, nbo_p, tagbits, nsub};
+               1ull << (tagbits - 1), merge ? 1u : 0u, fb, nbo_c, nbo_p, tagbits, nsub, range_inverse(nsub)};
   const uint32_t grid = std::min<uint32_t>(nsub, grid_limit);
   if (prev)
     hipLaunchKernelGGL(unique_diff_kernel<true>, dim3(grid), dim3(kUniqueThreads), 0, stream, a);

@@ -2,7 +2,9 @@
 // keyunmix(keymix(x)) == x for several key widths, and the top bits used
 // as bucket ids are near-uniform over all packed permutations of S9 and
 // over the children (swap of positions 0 and 1) of a sample of parents
-// spread across the mixed order.
+// spread across the mixed order.  Also range_start (the first top-32 value
+// of a sub-bucket, computed without a 64-bit division) against the exact
+// ceil(f 2^32 / n).
 #include <algorithm>
 #include <cstdio>
 #include <numeric>
@@ -19,8 +21,33 @@ static double max_over_mean(const std::vector<uint64_t>& keys, const KeyMix& km,
   return *std::max_element(count.begin(), count.end()) / mean;
 }
 
+static bool range_start_exact(uint32_t f, uint32_t n) {
+  const uint64_t want = ((static_cast<uint64_t>(f) << 32) + n - 1) / n;
+  return range_start(f, n, range_inverse(n)) == want;
+}
+
 int main() {
   std::mt19937_64 rng(7);
+  for (uint32_t n = 1; n <= 4096; ++n)
+    for (uint32_t f = 0; f < n; f += 1 + n / 97)
+      if (!range_start_exact(f, n) || !range_start_exact(n - 1, n)) {
+        std::printf("range_start failed n=%u f=%u\n", n, f);
+        return 1;
+      }
+  for (int i = 0; i < 2000000; ++i) {
+    const uint32_t n = 1 + static_cast<uint32_t>(rng() % (1u << 20));
+    const uint32_t f = static_cast<uint32_t>(rng() % n);
+    if (!range_start_exact(f, n)) {
+      std::printf("range_start failed n=%u f=%u\n", n, f);
+      return 1;
+    }
+  }
+  for (uint32_t n : {3u, 5120u, 131071u, 131072u, 1048575u, 1048576u})
+    for (uint32_t f : {0u, 1u, n / 2, n - 2, n - 1})
+      if (f < n && !range_start_exact(f, n)) {
+        std::printf("range_start failed n=%u f=%u\n", n, f);
+        return 1;
+      }
   for (uint32_t bits : {8u, 28u, 36u, 52u, 60u, 64u}) {
     const KeyMix km = make_keymix(bits);
     for (int i = 0; i < 200000; ++i) {
