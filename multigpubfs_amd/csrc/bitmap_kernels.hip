@@ -18,17 +18,21 @@ constexpr int kBitmapThreads = 256;
 constexpr int kChildrenPerThread = 24;
 
 template <int G>
-__global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
-    const uint64_t* __restrict__ frontier, uint64_t count, PackedGenerators gens,
+constexpr int bitmap_items() { return (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1; }
+
+// One tile of kBitmapThreads x kItems parents (tile index `tile_index`).
+template <int G>
+__device__ __forceinline__ void bitmap_tile(
+    uint64_t tile_index, const uint64_t* __restrict__ frontier, uint64_t count, const PackedGenerators& gens,
     uint32_t* __restrict__ bitmap, uint64_t bitmap_bits, uint64_t* __restrict__ next,
     unsigned long long* __restrict__ next_count, uint64_t count_base, uint64_t next_cap,
     uint32_t* __restrict__ error) {
-  constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
+  constexpr int kItems = bitmap_items<G>();
   __shared__ uint64_t parents[kItems][kBitmapThreads];
   __shared__ uint32_t wave_sums[kBitmapThreads / kWave];
   __shared__ unsigned long long wg_base;
   const uint32_t n = gens.degree;
-  const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kBitmapThreads * kItems;
+  const uint64_t tile = tile_index * kBitmapThreads * kItems;
   uint32_t newmask = 0;
   uint32_t errbits = 0;
 
@@ -115,6 +119,40 @@ __global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
   if (errbits) atomicOr(error, errbits);
 }
 
+template <int G>
+__global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_kernel(
+    const uint64_t* __restrict__ frontier, uint64_t count, PackedGenerators gens,
+    uint32_t* __restrict__ bitmap, uint64_t bitmap_bits, uint64_t* __restrict__ next,
+    unsigned long long* __restrict__ next_count, uint64_t count_base, uint64_t next_cap,
+    uint32_t* __restrict__ error) {
+  bitmap_tile<G>(blockIdx.x, frontier, count, gens, bitmap, bitmap_bits, next, next_count, count_base, next_cap,
+                 error);
+}
+
+// A layer whose size is read from the device, so layers can be queued
+// without the host reading each size: marks[0] and marks[1] are the layer
+// counter before and after the previous layer was produced, so the frontier
+// holds marks[1] - marks[0] states and the children are counted from
+// marks[1].  A fixed grid walks the tiles.
+template <int G>
+__global__ __launch_bounds__(kBitmapThreads) void bitmap_expand_marked_kernel(
+    const uint64_t* __restrict__ frontier, const unsigned long long* __restrict__ marks, PackedGenerators gens,
+    uint32_t* __restrict__ bitmap, uint64_t bitmap_bits, uint64_t* __restrict__ next,
+    unsigned long long* __restrict__ next_count, uint64_t next_cap, uint32_t* __restrict__ error) {
+  const uint64_t base = marks[1], count = base - marks[0];
+  const uint64_t per_block = static_cast<uint64_t>(kBitmapThreads) * bitmap_items<G>();
+  const uint64_t tiles = (count + per_block - 1) / per_block;
+  for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    bitmap_tile<G>(t, frontier, count, gens, bitmap, bitmap_bits, next, next_count, base, next_cap, error);
+    __syncthreads();  // the tile's shared arrays are reused by the next one
+  }
+}
+
+// mark = the layer counter once the layers before it on the stream are done.
+__global__ void bitmap_mark_kernel(const unsigned long long* __restrict__ counter, unsigned long long* __restrict__ mark) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *mark = *counter;
+}
+
 __global__ void bitmap_seed_kernel(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits,
                                    uint64_t* frontier, uint32_t* error) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -132,7 +170,7 @@ template <int G>
 static hipError_t launch_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens,
                                 uint32_t* bitmap, uint64_t bits, uint64_t* next, unsigned long long* next_count,
                                 uint64_t count_base, uint64_t next_cap, uint32_t* error, hipStream_t stream) {
-  constexpr int kItems = (kChildrenPerThread / G) > 0 ? (kChildrenPerThread / G) : 1;
+  constexpr int kItems = bitmap_items<G>();
   const uint64_t per_block = static_cast<uint64_t>(kBitmapThreads) * kItems;
   const uint64_t blocks = (count + per_block - 1) / per_block;
   if (blocks == 0) return hipSuccess;
@@ -156,6 +194,31 @@ hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedG
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t bitmap_expand_marked(const uint64_t* frontier, const unsigned long long* marks,
+                                const PackedGenerators& gens, uint32_t* bitmap, uint64_t bitmap_bits, uint64_t* next,
+                                unsigned long long* next_count, uint64_t next_cap, uint32_t* error, uint32_t grid,
+                                hipStream_t stream) {
+  if (grid == 0) return hipErrorInvalidValue;
+  switch (gens.count) {
+#define MGBFS_G(k)                                                                                             \
+  case k:                                                                                                      \
+    hipLaunchKernelGGL(bitmap_expand_marked_kernel<k>, dim3(grid), dim3(kBitmapThreads), 0, stream, frontier,   \
+                       marks, gens, bitmap, bitmap_bits, next, next_count, next_cap, error);                    \
+    break;
+    MGBFS_G(1) MGBFS_G(2) MGBFS_G(3) MGBFS_G(4) MGBFS_G(5) MGBFS_G(6) MGBFS_G(7) MGBFS_G(8)
+    MGBFS_G(9) MGBFS_G(10) MGBFS_G(11) MGBFS_G(12) MGBFS_G(13) MGBFS_G(14) MGBFS_G(15) MGBFS_G(16)
+#undef MGBFS_G
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t bitmap_mark(const unsigned long long* counter, unsigned long long* mark, hipStream_t stream) {
+  hipLaunchKernelGGL(bitmap_mark_kernel, dim3(1), dim3(64), 0, stream, counter, mark);
+  return hipGetLastError();
 }
 
 hipError_t bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, uint64_t bitmap_bits, uint64_t* frontier,

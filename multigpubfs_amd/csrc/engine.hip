@@ -118,7 +118,9 @@ class BitmapEngine final : public Engine {
     bitmap_.allocate(bitmap_bytes);
     control_.allocate(64);
     stash_.allocate(kStashRows * 8);
+    marks_.allocate((kAhead + 2) * 8);
     MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), 64, hipHostMallocDefault));
+    MGBFS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_marks_), (kAhead + 2) * 8, hipHostMallocDefault));
     cap_limit_ = o.capacity ? o.capacity : domain_;
     // Frontier buffers grow on demand up to what the budget admits.
     const uint64_t frontier_budget = (budget_ - bitmap_bytes) / 2 / 8;
@@ -153,6 +155,7 @@ class BitmapEngine final : public Engine {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (host_ctl_) (void)hipHostFree(host_ctl_);
+    if (host_marks_) (void)hipHostFree(host_marks_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -225,6 +228,79 @@ class BitmapEngine final : public Engine {
     return MGBFS_LAYER;
   }
 
+  // Runs to the end with up to kAhead layers queued per host wait, when
+  // both frontier buffers can hold every state not yet reached (so no layer
+  // can outgrow them, and none is chunked): each layer reads its size and
+  // counter base from marks_ (the running counter before and after the
+  // layer that produced it) and a one-thread kernel records the counter
+  // after it.  Otherwise, and for bigger state spaces, one layer per wait.
+  int run_to_end() override {
+    if (terminal) return MGBFS_LAYER;
+    const uint64_t remaining = domain_ - total();
+    if (remaining > cap_limit_ || remaining > kAheadStates) return Engine::run_to_end();
+    MGBFS_HIP(hipSetDevice(device_));
+    if (next_cap_ < remaining) grow_next(remaining);
+    if (front_cap_ < remaining) grow_front(remaining);
+    update_bytes();
+    stash_count_ = 0;  // no layer here is chunked
+    int status = MGBFS_LAYER;
+    while (!terminal) {
+      host_marks_[0] = counter_base_ - count_;
+      host_marks_[1] = counter_base_;
+      MGBFS_HIP(hipMemcpyAsync(marks_.as<uint64_t>(), host_marks_, 16, hipMemcpyHostToDevice, stream_));
+      MGBFS_HIP(hipEventRecord(ev0_, stream_));
+      auto* marks = reinterpret_cast<unsigned long long*>(marks_.as<uint64_t>());
+      auto* counter = reinterpret_cast<unsigned long long*>(control_.as<uint64_t>());
+      for (uint32_t i = 0; i < kAhead; ++i) {
+        const bool even = (i & 1u) == 0;
+        uint64_t* src = (even ? front_ : next_).as<uint64_t>();
+        uint64_t* dst = (even ? next_ : front_).as<uint64_t>();
+        MGBFS_HIP(bitmap_expand_marked(src, marks + i, gens_, bitmap_.as<uint32_t>(), domain_, dst, counter,
+                                       remaining, error_word(), kAheadGrid, stream_));
+        MGBFS_HIP(bitmap_mark(counter, marks + i + 2, stream_));
+      }
+      MGBFS_HIP(hipEventRecord(ev1_, stream_));
+      MGBFS_HIP(hipMemcpyAsync(host_marks_, marks, (kAhead + 2) * 8, hipMemcpyDeviceToHost, stream_));
+      read_control();  // queued behind the layers: one wait for all of them
+      float ms = 0.f;
+      MGBFS_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+      kernel_seconds += ms * 1e-3;
+      const uint32_t err = static_cast<uint32_t>(host_ctl_[1]);
+      if (err) {
+        terminal = true;
+        return MGBFS_RESOURCE;
+      }
+      // Layer i + 1 of the batch holds marks[i + 2] - marks[i + 1] states;
+      // the batch ends at the first empty one.
+      uint32_t produced = 0;
+      for (uint32_t i = 0; i < kAhead; ++i) {
+        const uint64_t parents = host_marks_[i + 1] - host_marks_[i];
+        const uint64_t kids = host_marks_[i + 2] - host_marks_[i + 1];
+        ++launches;
+        candidates += parents * gens_.count;
+        streamed_bytes += 8 * parents + 8 * kids;
+        if (kids == 0) {
+          terminal = true;
+          complete_ = true;
+          status = MGBFS_COMPLETE;
+          break;
+        }
+        layers.push_back(kids);
+        prev_count_ = parents;
+        count_ = kids;
+        ++produced;
+      }
+      // Layer k of the batch sits in front_ when k is even, else in next_.
+      if (produced & 1u) {
+        std::swap(front_, next_);
+        std::swap(front_cap_, next_cap_);
+      }
+      counter_base_ = host_marks_[produced + 1];
+      if (!terminal) status = MGBFS_LAYER;
+    }
+    return status;
+  }
+
   // Another graph of the same degree reuses the bitmap and frontiers.
   void rebind(const Graph& g) override {
     if (g.elements != n_ || !g.packable() || !g.distinct_perm()) throw std::runtime_error("ENGINE_REBIND_SHAPE");
@@ -282,6 +358,17 @@ class BitmapEngine final : public Engine {
   }
   // Grows next_ (which holds the previous layer between advances) keeping
   // the first rows that the previous-layer sample can still ask for.
+  // Grows front_ keeping the current layer.
+  void grow_front(uint64_t want) {
+    DeviceBuffer fresh;
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    fresh.allocate(want * 8);
+    if (count_)
+      MGBFS_HIP(hipMemcpyAsync(fresh.as<uint64_t>(), front_.as<uint64_t>(), count_ * 8, hipMemcpyDeviceToDevice, stream_));
+    MGBFS_HIP(hipStreamSynchronize(stream_));
+    front_ = std::move(fresh);
+    front_cap_ = want;
+  }
   void grow_next(uint64_t want) {
     DeviceBuffer fresh;
     MGBFS_HIP(hipStreamSynchronize(stream_));
@@ -303,21 +390,28 @@ class BitmapEngine final : public Engine {
     }
   }
   void update_bytes() {
-    device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes() + stash_.bytes();
+    device_bytes = bitmap_.bytes() + front_.bytes() + next_.bytes() + control_.bytes() + stash_.bytes() + marks_.bytes();
   }
 
   // Rows of the previous layer kept across an advance (the states.json
   // previous-layer sample holds at most 1000 rows).
   static constexpr uint64_t kStashRows = 1000;
+  // Layers queued per host wait by run_to_end, the fixed grid of those
+  // layers, and the largest unreached state count it runs with (two
+  // frontier buffers of that many states: 2 GiB).
+  static constexpr uint32_t kAhead = 16;
+  static constexpr uint32_t kAheadGrid = 1024;
+  static constexpr uint64_t kAheadStates = 1ull << 27;
   PackedGenerators gens_;
   uint32_t n_;
   uint64_t domain_ = 0, budget_ = 0, cap_limit_ = 0;
-  DeviceBuffer bitmap_, front_, next_, control_, stash_;
+  DeviceBuffer bitmap_, front_, next_, control_, stash_, marks_;
   uint64_t front_cap_ = 0, next_cap_ = 0, count_ = 0, prev_count_ = 0, stash_count_ = 0, start_key_ = 0,
            key_mask_ = ~0ull;
   uint64_t counter_base_ = 0;  // value of the running child counter at the start of the layer
   bool complete_ = false;
   uint64_t* host_ctl_ = nullptr;
+  uint64_t* host_marks_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };
 
@@ -701,6 +795,11 @@ extern "C" int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_se
   if (!engine) return hipErrorInvalidValue;
   MGBFS_GUARD({
     Engine* e = static_cast<Engine*>(engine);
+    if (!max_layers && max_seconds <= 0 && !e->terminal) {
+      const int s = e->run_to_end();
+      e->settle();
+      return s;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     int s = MGBFS_LAYER;
     uint32_t done = 0;

@@ -74,6 +74,18 @@ class Engine {
  public:
   virtual ~Engine() = default;
   virtual int advance(uint64_t* next_count, uint32_t* fatal) = 0;
+  // Runs to the end (no layer or time limit): advance() until terminal,
+  // returning the last status.  A backend may queue several layers per
+  // host wait instead.
+  virtual int run_to_end() {
+    int s = 0;
+    while (!terminal) {
+      uint64_t nc;
+      uint32_t f;
+      s = advance(&nc, &f);
+    }
+    return s;
+  }
   virtual void reset() = 0;
   // Switches to another graph of the same shape, keeping every allocation,
   // and resets (backends that support it).

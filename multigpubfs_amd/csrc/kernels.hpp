@@ -13,6 +13,14 @@ hipError_t bitmap_seed(uint32_t degree, uint64_t start_key, uint32_t* bitmap, ui
 hipError_t bitmap_expand(const uint64_t* frontier, uint64_t count, const PackedGenerators& gens, uint32_t* bitmap,
                          uint64_t bitmap_bits, uint64_t* next, unsigned long long* next_count, uint64_t count_base,
                          uint64_t next_cap, uint32_t* error, hipStream_t stream);
+// The same layer with its size and counter base read from the device
+// (marks[0], marks[1]: the counter before and after the previous layer), on
+// a fixed grid; bitmap_mark then stores the counter into the next mark.
+hipError_t bitmap_expand_marked(const uint64_t* frontier, const unsigned long long* marks,
+                                const PackedGenerators& gens, uint32_t* bitmap, uint64_t bitmap_bits, uint64_t* next,
+                                unsigned long long* next_count, uint64_t next_cap, uint32_t* error, uint32_t grid,
+                                hipStream_t stream);
+hipError_t bitmap_mark(const unsigned long long* counter, unsigned long long* mark, hipStream_t stream);
 // Sorted-bank backend limits.
 constexpr uint32_t kMaxPassBits = 10;     // buckets per routing pass (LDS counters)
 constexpr uint32_t kMaxBucketBits = 20;   // sub-buckets per layer: 2^20
