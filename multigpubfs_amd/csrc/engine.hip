@@ -400,7 +400,7 @@ class BitmapEngine final : public Engine {
   // layers, and the largest unreached state count it runs with (two
   // frontier buffers of that many states: 2 GiB).
   static constexpr uint32_t kAhead = 16;
-  static constexpr uint32_t kAheadGrid = 1024;
+  static constexpr uint32_t kAheadGrid = 2048;
   static constexpr uint64_t kAheadStates = 1ull << 27;
   PackedGenerators gens_;
   uint32_t n_;
