@@ -143,6 +143,7 @@ class BitmapEngine final : public Engine {
     counter_base_ = 0;
     prev_count_ = stash_count_ = 0;
     complete_ = false;
+    batch_failed_ = false;
     layers.assign(1, 1);
     candidates = unique_candidates = 0;
     kernel_seconds = 0.0;
@@ -267,7 +268,10 @@ class BitmapEngine final : public Engine {
       kernel_seconds += ms * 1e-3;
       const uint32_t err = static_cast<uint32_t>(host_ctl_[1]);
       if (err) {
+        // Later layers of the batch may have overwritten both frontiers:
+        // no samples from here on.
         terminal = true;
+        batch_failed_ = true;
         return MGBFS_RESOURCE;
       }
       // Layer i + 1 of the batch holds marks[i + 2] - marks[i + 1] states;
@@ -308,11 +312,18 @@ class BitmapEngine final : public Engine {
     reset();
   }
 
-  void layer_keys(std::vector<uint64_t>& out) override { copy_keys(front_, count_, count_, out); }
+  void layer_keys(std::vector<uint64_t>& out) override {
+    if (batch_failed_)
+      out.clear();
+    else
+      copy_keys(front_, count_, count_, out);
+  }
 
   void sample(uint32_t which, uint64_t limit, std::vector<int64_t>& rows, uint64_t& count) override {
     std::vector<uint64_t> keys;
-    if (which == 0) {
+    if (batch_failed_) {
+      // Neither frontier holds a known layer.
+    } else if (which == 0) {
       copy_keys(front_, count_, limit, keys);
     } else if (terminal && !complete_) {
       // A failed advance wrote children over next_: the stash holds the
@@ -410,6 +421,7 @@ class BitmapEngine final : public Engine {
            key_mask_ = ~0ull;
   uint64_t counter_base_ = 0;  // value of the running child counter at the start of the layer
   bool complete_ = false;
+  bool batch_failed_ = false;  // run_to_end stopped on a device error
   uint64_t* host_ctl_ = nullptr;
   uint64_t* host_marks_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
