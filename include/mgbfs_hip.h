@@ -150,7 +150,9 @@ int mgbfs_engine_create(const mgbfs_graph_desc* graph, const mgbfs_engine_option
  * failure.  *next_count receives the size of the new current layer. */
 int mgbfs_engine_advance(void* engine, uint64_t* next_count, uint32_t* fatal);
 /* Runs advance() until completion, a resource stop, max_layers layers or
- * max_seconds (0 = unlimited).  Returns the last advance() status. */
+ * max_seconds (0 = unlimited).  Returns the last advance() status.  With
+ * neither limit the permutation-bitmap backend may queue several layers per
+ * host wait (same layers, same final state). */
 int mgbfs_engine_run(void* engine, uint32_t max_layers, double max_seconds);
 int mgbfs_engine_query(void* engine, mgbfs_engine_info* info);
 /* Per-phase device seconds and modelled streaming HBM bytes of the hot
