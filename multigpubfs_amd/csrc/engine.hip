@@ -526,7 +526,7 @@ uint64_t state_space_bound(const Graph& g) {
 static uint32_t choose_backend(const Graph& g, uint32_t requested) {
   uint32_t backend = requested;
   if (backend == MGBFS_BACKEND_AUTO) {
-    // The sorted backend keeps at least 4 tag bits beside the 4n key bits;
+    // The sorted backend keeps at least 4 tag bits beside its key bits;
     // other inverse-closed graphs keep three rolling banks in segmented
     // tables; other directed graphs keep their full history.
     if (g.packable() && g.inverse_closed && g.elements <= 15) backend = MGBFS_BACKEND_SORTED;
