@@ -290,6 +290,60 @@ def test_samples_match_last_and_previous_layers(backend):
     assert sorted(O.pack(r) for r in prev) == sorted(ref["keys"][-2])
 
 
+# run() with no layer or time limit lets the permutation-bitmap backend queue
+# up to 16 layers per host wait (layer sizes read from device counter marks):
+# the tables, the last two layers and resumed runs must match one advance()
+# per layer and the oracle.
+def test_bitmap_run_to_end_matches_golden_table_and_layer_loop():
+    with NativeEngine(lrx(10), backend="permutation_bitmap") as e:
+        assert e.run() == 1
+        sizes = e.layer_sizes()
+        total = e.info()["total_states"]
+        last = sorted(e.layer_keys())
+    with NativeEngine(lrx(10), backend="permutation_bitmap") as e:
+        stepped, _ = run_all_layers(e)
+        stepped_last = sorted(e.layer_keys())
+    assert sizes == stepped == GOLDEN["layers"]["10"]
+    assert total == math.factorial(10)
+    assert last == stepped_last
+
+
+def test_bitmap_run_to_end_samples_match_last_and_previous_layers():
+    ref = O.packed_bfs(8, keep_keys=True)
+    with NativeEngine(lrx(8), backend="permutation_bitmap") as e:
+        e.run()
+        cur = e.sample(0, 1000)
+        prev = e.sample(1, 1000)
+    assert sorted(O.pack(r) for r in cur) == sorted(ref["keys"][-1])
+    assert sorted(O.pack(r) for r in prev) == sorted(ref["keys"][-2])
+
+
+def test_bitmap_run_to_end_after_advances_and_reset():
+    ref = O.packed_bfs(9)["layers"]
+    with NativeEngine(lrx(9), backend="permutation_bitmap") as e:
+        for _ in range(5):
+            assert e.advance() == 0
+        e.run()
+        assert e.layer_sizes() == ref
+        e.reset()
+        e.run()
+        assert e.layer_sizes() == ref
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_bitmap_run_to_end_directed_sets_match_oracle(seed):
+    rng = random.Random(seed)
+    n = rng.choice([7, 8, 9])
+    gens = random_generators(rng, n, rng.randint(2, 5), False)
+    ref = O.packed_bfs(n, gens, keep_keys=True)
+    with NativeEngine(GraphDefinition.permutation(gens, list(range(n))), backend="permutation_bitmap") as e:
+        e.run()
+        sizes = e.layer_sizes()
+        last = sorted(e.layer_keys())
+    assert sizes == ref["layers"]
+    assert last == sorted(ref["keys"][-1])
+
+
 @pytest.mark.parametrize("backend", BACKENDS)
 def test_reset_repeats_identical_tables(backend):
     with NativeEngine(lrx(9), backend=backend) as e:
