@@ -105,6 +105,10 @@ class NativeEngine:
         desc = graph_desc(graph)
         self.elements = desc.elements
         if inverse_closed is not None:
+            # Override on a copy: the graph's own descriptor stays as validated.
+            keep = desc._keep
+            desc = GraphDesc.from_buffer_copy(desc)
+            desc._keep = keep
             desc.inverse_closed = int(inverse_closed)
         self._keep = [desc]
         opts = EngineOptions()
@@ -301,9 +305,15 @@ def _bind_batch(lib):
 
 
 def graph_desc(graph):
-    """mgbfs_graph_desc for a GraphDefinition (arrays kept alive on the object)."""
+    """mgbfs_graph_desc for a GraphDefinition (arrays kept alive on the
+    descriptor).  A GraphDefinition's contents never change after validation,
+    so its descriptor is built once and kept on it; the library copies what
+    it needs from a descriptor during each call."""
     from .graph_definition import GraphDefinition
     gd = graph if isinstance(graph, GraphDefinition) else GraphDefinition(graph)
+    cached = gd.__dict__.get("_desc")
+    if cached is not None:
+        return cached
     data = gd._data  # read only
     a = data["action"]
     desc = GraphDesc()
@@ -324,6 +334,7 @@ def graph_desc(graph):
         keep.append(_array(C.c_uint32, [g["modulo"] for g in a["generators"]]))
         desc.moduli = keep[-1]
     desc._keep = keep
+    gd._desc = desc
     return desc
 
 

@@ -9,7 +9,10 @@ inverse-closed), odd ones do not (directed, full visited history).
 `run_batched` advances all instances together through the batch engine
 (hipGraph-captured layer loop); `run_sequential` runs the same instances one
 at a time through the plain single-instance engine.  Both return per-instance
-layer tables, which must be identical.
+layer tables, which must be identical.  Every leg validates its instances
+and builds their library descriptors before its clock starts, so the timed
+region holds the same work in each: engine set-up, BFS and reading the
+tables.
 
 The reference has no batched kernel; its sweeps run instances one by one
 through resident workers (docs/reproduce-pair-sweeps.md:3,
@@ -74,6 +77,15 @@ def make_sweep(count: int = 256) -> list[dict]:
     return [make_instance(i) for i in range(count)]
 
 
+def prepare(instances) -> list[GraphDefinition]:
+    """Validated GraphDefinitions with their library descriptors built."""
+    from ._native import graph_desc
+    graphs = [g if isinstance(g, GraphDefinition) else GraphDefinition(g) for g in instances]
+    for g in graphs:
+        graph_desc(g)
+    return graphs
+
+
 # BatchEngine options an autotuned profile may set (mgbfs_batch_options).
 ENGINE_KEYS = ("threads", "hash_load_pct", "history", "layer_pct", "bitmap_bias_permille")
 
@@ -96,13 +108,14 @@ def run_batched(instances, *, device=0, max_slots=None, visited="auto", layers=1
     options.update(engine)
     modes = {}
     restarts = 0
+    graphs = prepare(instances)
     with BatchEngine(device=device, max_slots=slots, grid=grid, layers_per_graph=layers_per_graph, **options) as b:
         t0 = time.perf_counter()
-        pending = list(range(len(instances)))
+        pending = list(range(len(graphs)))
         owner = {}
         while pending or owner:
             while pending:
-                slot = b.admit(instances[pending[0]], visited=visited)
+                slot = b.admit(graphs[pending[0]], visited=visited)
                 if slot is None:
                     break
                 owner[slot] = pending.pop(0)
@@ -126,10 +139,10 @@ def run_sequential(instances, *, device=0, backend="permutation_bitmap"):
     destroy).  The Lehmer-bitmap backend is the plain engine's fastest exact
     choice for these S9/S10 instances, directed or not."""
     from ._native import NativeEngine
+    graphs = prepare(instances)
     tables = []
     t0 = time.perf_counter()
-    for g in instances:
-        gd = GraphDefinition(g)
+    for gd in graphs:
         with NativeEngine(gd, device=device, backend=backend) as e:
             status = e.run()
             if status != 1:
@@ -144,17 +157,17 @@ def run_sequential_reusing(instances, *, device=0, backend="permutation_bitmap")
     the timed region), and each instance swaps its generators into that
     engine, which resets its bitmap and frontiers in place."""
     from ._native import NativeEngine
+    graphs = prepare(instances)
     engines = {}
-    for g in instances:
-        n = g["action"]["degree"]
-        if n not in engines:
-            engines[n] = NativeEngine(GraphDefinition(g), device=device, backend=backend)
+    for g in graphs:
+        if g.state_elements not in engines:
+            engines[g.state_elements] = NativeEngine(g, device=device, backend=backend)
     tables = []
     try:
         t0 = time.perf_counter()
-        for g in instances:
-            e = engines[g["action"]["degree"]]
-            e.rebind(GraphDefinition(g))
+        for g in graphs:
+            e = engines[g.state_elements]
+            e.rebind(g)
             status = e.run()
             if status != 1:
                 raise RuntimeError(f"SEQUENTIAL_INSTANCE_STATUS {status}")
