@@ -105,11 +105,7 @@ class NativeEngine:
         desc = graph_desc(graph)
         self.elements = desc.elements
         if inverse_closed is not None:
-            # Override on a copy: the graph's own descriptor stays as validated.
-            keep = desc._keep
-            desc = GraphDesc.from_buffer_copy(desc)
-            desc._keep = keep
-            desc.inverse_closed = int(inverse_closed)
+            desc = desc_with_inverse_closed(desc, inverse_closed)
         self._keep = [desc]
         opts = EngineOptions()
         opts.device = device
@@ -302,6 +298,15 @@ def _bind_batch(lib):
         getattr(lib, "mgbfs_batch_" + name).restype = C.c_int
     lib._batch_bound = True
     return lib
+
+
+def desc_with_inverse_closed(desc, inverse_closed):
+    """A copy of `desc` with its inverse-closure flag overridden; the graph's
+    own (cached) descriptor stays as validated.  The copy shares the arrays."""
+    out = GraphDesc.from_buffer_copy(desc)
+    out._keep = desc._keep
+    out.inverse_closed = int(inverse_closed)
+    return out
 
 
 def graph_desc(graph):

@@ -324,6 +324,27 @@ def test_sweep_instances_are_deterministic_and_shaped():
         assert (2 <= k <= 5) if i % 2 else (4 <= k <= 10 and gd.inverse_closed())
 
 
+def test_graph_descriptor_is_built_once_and_overrides_copy_it():
+    from multigpubfs_amd._native import desc_with_inverse_closed, graph_desc
+    from multigpubfs_amd.sweep import make_sweep, prepare
+    graphs = make_sweep(4)
+    prepared = prepare(graphs)
+    for g, gd in zip(graphs, prepared):
+        d = graph_desc(gd)
+        assert graph_desc(gd) is d  # kept on the validated graph
+        assert graph_desc(g) is not d  # a plain dict is validated afresh
+        n, k = gd.state_elements, gd.generator_count
+        assert (d.kind, d.elements, d.rows, d.cols, d.generators) == (0, n, n, 1, k)
+        assert d.inverse_closed == int(gd.inverse_closed())
+        assert list(d.start[:n]) == gd.start
+        assert list(d.permutations[:n * k]) == [v for p in gd.action["generators"] for v in p]
+        flipped = desc_with_inverse_closed(d, not gd.inverse_closed())
+        assert flipped.inverse_closed == 1 - d.inverse_closed
+        assert d.inverse_closed == int(gd.inverse_closed())  # the cached one is untouched
+        assert list(flipped.permutations[:n * k]) == list(d.permutations[:n * k])
+    assert prepare(prepared)[0] is prepared[0]
+
+
 def test_serving_job_stream_mix_and_rate():
     from multigpubfs_amd.serving import make_jobs
     jobs = make_jobs(2000)
