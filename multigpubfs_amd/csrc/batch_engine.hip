@@ -290,13 +290,11 @@ class BatchEngine {
     if (p.visited == kVisBitmap) {
       d.bitmap = reinterpret_cast<uint32_t*>(visited_base);
       d.bitmap_bits = factorial_u64(g.elements);
-      MGBFS_HIP(fill_u32(d.bitmap, 0u, p.visited_bytes / 4, stream_));
     } else {
       d.table = reinterpret_cast<uint64_t*>(visited_base);
       d.table_slots = p.table_slots;
       d.table_stride = p.table_slots;
       d.rolling = p.rolling ? 1u : 0u;
-      MGBFS_HIP(fill_u64(d.table, ~0ull, p.visited_bytes / 8, stream_));
     }
     d.tag_shift = 64;
     if (g.kind == 0) {
@@ -326,8 +324,8 @@ class BatchEngine {
           d.mat[k][e] = static_cast<int32_t>(g.matrices[static_cast<size_t>(k) * g.rows * g.rows + e]);
       for (uint32_t e = 0; e < g.elements; ++e) start_key |= static_cast<uint64_t>(g.start[e]) << (e * eb);
     }
-    // The descriptor is installed and seeded by the next flush(), which the
-    // next step runs before its graph (stream order puts the fill first).
+    // The visited set is emptied, and the descriptor installed and seeded,
+    // by the next flush(), which the next step runs before its graph.
     pending_[slot] = kInstallSeed;
     pending_key_[slot] = start_key;
     sl.used = true;
@@ -462,8 +460,12 @@ class BatchEngine {
     MGBFS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
   }
   // Installs every descriptor admitted or released since the last flush
-  // (the latest content per slot) with one upload and one kernel.  The
-  // pinned staging area is reused only after refresh() has waited.
+  // (the latest content per slot) with one upload and two kernels: the
+  // visited sets of the admitted ones are emptied, then the descriptors are
+  // installed and seeded.  Only the latest descriptor of a slot is staged,
+  // so memory an instance released before the flush is cleared, if reused,
+  // only as its new owner's visited set.  The pinned staging area is reused
+  // only after refresh() has waited.
   void flush() {
     uint32_t n = 0;
     BatchInstall* req = reinterpret_cast<BatchInstall*>(host_stage_ + slots_n_);
@@ -478,8 +480,9 @@ class BatchEngine {
     const size_t bytes = sizeof(BatchInstance) * slots_n_ + sizeof(BatchInstall) * n;
     MGBFS_HIP(hipMemcpyAsync(stage_.as<void>(), host_stage_, bytes, hipMemcpyHostToDevice, stream_));
     const BatchInstance* staged = stage_.as<BatchInstance>();
-    MGBFS_HIP(batch_install(inst_.as<BatchInstance>(), staged,
-                            reinterpret_cast<const BatchInstall*>(staged + slots_n_), n, stream_));
+    const BatchInstall* reqs = reinterpret_cast<const BatchInstall*>(staged + slots_n_);
+    MGBFS_HIP(batch_clear_visited(staged, reqs, n, stream_));
+    MGBFS_HIP(batch_install(inst_.as<BatchInstance>(), staged, reqs, n, stream_));
   }
   void refresh() {
     MGBFS_HIP(hipMemcpyAsync(host_mirror_, inst_.as<void>(), sizeof(BatchInstance) * slots_n_, hipMemcpyDeviceToHost,

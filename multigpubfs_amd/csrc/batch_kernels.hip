@@ -434,6 +434,28 @@ __global__ void batch_seed_kernel(BatchInstance* inst, uint32_t slot, uint64_t s
   if (threadIdx.x == 0) seed_instance(inst[slot], start_key);
 }
 
+// Blocks (x, y) clear the visited set of request y (and y + gridDim.y ...)
+// when it seeds: a Lehmer bitmap of ceil(n! / 32) words to 0, or the hash
+// table(s), (rolling ? 3 : 1) x table_slots slots, to ~0.
+__global__ __launch_bounds__(256) void batch_clear_visited_kernel(const BatchInstance* staged,
+                                                                   const BatchInstall* req, uint32_t n) {
+  for (uint32_t q = blockIdx.y; q < n; q += gridDim.y) {
+    if (!req[q].seed) continue;
+    const BatchInstance& d = staged[q];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (d.visited == kVisBitmap) {
+      const global_ptr<uint32_t> bitmap = gp(d.bitmap);
+      const uint64_t words = (d.bitmap_bits + 31) / 32;
+      for (uint64_t k = first; k < words; k += stride) bitmap[k] = 0u;
+    } else {
+      const global_ptr<uint64_t> table = gp(d.table);
+      const uint64_t slots = (d.rolling ? 3ull : 1ull) * d.table_slots;
+      for (uint64_t k = first; k < slots; k += stride) table[k] = ~0ull;
+    }
+  }
+}
+
 // Block q writes staged descriptor q into its slot and, when asked, seeds it.
 __global__ void batch_install_kernel(BatchInstance* inst, const BatchInstance* staged, const BatchInstall* req,
                                      uint32_t n) {
@@ -504,6 +526,13 @@ hipError_t batch_install(BatchInstance* inst, const BatchInstance* staged, const
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(batch_install_kernel, dim3(std::min<uint32_t>(n, 1024)), dim3(64), 0, stream, inst, staged, req,
                      n);
+  return hipGetLastError();
+}
+
+hipError_t batch_clear_visited(const BatchInstance* staged, const BatchInstall* req, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_clear_visited_kernel, dim3(32, std::min<uint32_t>(n, 1024)), dim3(256), 0, stream, staged,
+                     req, n);
   return hipGetLastError();
 }
 
