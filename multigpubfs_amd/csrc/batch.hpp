@@ -85,7 +85,9 @@ hipError_t batch_gather_sizes(const BatchInstance* inst, const uint32_t* req, ui
                               hipStream_t stream);
 // Empties the visited set of every seeding request's staged descriptor
 // (bitmap words to 0, hash slots to ~0); runs before batch_install.
-hipError_t batch_clear_visited(const BatchInstance* staged, const BatchInstall* req, uint32_t n, hipStream_t stream);
+// max_bytes: the largest of those visited sets (sizes the grid).
+hipError_t batch_clear_visited(const BatchInstance* staged, const BatchInstall* req, uint32_t n, uint64_t max_bytes,
+                               hipStream_t stream);
 hipError_t batch_install(BatchInstance* inst, const BatchInstance* staged, const BatchInstall* req, uint32_t n,
                          hipStream_t stream);
 hipError_t batch_seed(BatchInstance* inst, uint32_t slot, uint64_t start_key, hipStream_t stream);

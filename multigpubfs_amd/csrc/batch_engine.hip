@@ -468,11 +468,17 @@ class BatchEngine {
   // only after refresh() has waited.
   void flush() {
     uint32_t n = 0;
+    uint64_t clear_bytes = 0;  // largest visited set to empty
     BatchInstall* req = reinterpret_cast<BatchInstall*>(host_stage_ + slots_n_);
     for (uint32_t s = 0; s < slots_n_; ++s) {
       if (pending_[s] == kNothing) continue;
-      host_stage_[n] = host_inst_[s];
+      const BatchInstance& d = host_inst_[s];
+      host_stage_[n] = d;
       req[n] = BatchInstall{s, pending_[s] == kInstallSeed ? 1u : 0u, pending_key_[s]};
+      if (req[n].seed)
+        clear_bytes = std::max<uint64_t>(clear_bytes, d.visited == kVisBitmap
+                                                          ? (d.bitmap_bits + 31) / 32 * 4
+                                                          : (d.rolling ? 3ull : 1ull) * d.table_slots * 8);
       pending_[s] = kNothing;
       ++n;
     }
@@ -481,7 +487,7 @@ class BatchEngine {
     MGBFS_HIP(hipMemcpyAsync(stage_.as<void>(), host_stage_, bytes, hipMemcpyHostToDevice, stream_));
     const BatchInstance* staged = stage_.as<BatchInstance>();
     const BatchInstall* reqs = reinterpret_cast<const BatchInstall*>(staged + slots_n_);
-    MGBFS_HIP(batch_clear_visited(staged, reqs, n, stream_));
+    MGBFS_HIP(batch_clear_visited(staged, reqs, n, clear_bytes, stream_));
     MGBFS_HIP(batch_install(inst_.as<BatchInstance>(), staged, reqs, n, stream_));
   }
   void refresh() {

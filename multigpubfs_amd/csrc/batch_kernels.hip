@@ -529,9 +529,12 @@ hipError_t batch_install(BatchInstance* inst, const BatchInstance* staged, const
   return hipGetLastError();
 }
 
-hipError_t batch_clear_visited(const BatchInstance* staged, const BatchInstall* req, uint32_t n, hipStream_t stream) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_clear_visited_kernel, dim3(32, std::min<uint32_t>(n, 1024)), dim3(256), 0, stream, staged,
+hipError_t batch_clear_visited(const BatchInstance* staged, const BatchInstall* req, uint32_t n, uint64_t max_bytes,
+                               hipStream_t stream) {
+  if (n == 0 || max_bytes == 0) return hipSuccess;
+  // About 64 KB of the largest set per workgroup (256 bytes per thread).
+  const uint32_t x = static_cast<uint32_t>(std::min<uint64_t>(1024, (max_bytes + 65535) / 65536));
+  hipLaunchKernelGGL(batch_clear_visited_kernel, dim3(x, std::min<uint32_t>(n, 1024)), dim3(256), 0, stream, staged,
                      req, n);
   return hipGetLastError();
 }
