@@ -72,6 +72,29 @@ def test_continuous_admission_results_equal_alone():
     assert [done[i] for i in range(len(jobs))] == alone
 
 
+def test_memory_reused_before_a_step_starts_empty_for_its_new_owner():
+    # A released block, taken again before the next step, still holds the old
+    # instance's frontier keys and visited bits (or a pending, never-run
+    # instance's nothing); the new owner's hash table or bitmap must start
+    # empty all the same, and an instance released before any step must not
+    # clear memory someone else now owns.
+    first, second, dropped, last = GOLDEN[0], GOLDEN[2], GOLDEN[4], GOLDEN[6]
+    with BatchEngine(max_slots=2) as b:
+        s = b.admit(first["graph"], visited="bitmap")
+        b.run()
+        sizes, _, error = b.result(s)
+        assert error == 0 and sizes == first["layer_sizes"]
+        b.release(s)
+        s1 = b.admit(second["graph"], visited="hash")
+        s2 = b.admit(dropped["graph"], visited="hash")
+        b.release(s2)
+        s2 = b.admit(last["graph"], visited="bitmap")
+        b.run()
+        for slot, want in ((s1, second), (s2, last)):
+            sizes, _, error = b.result(slot)
+            assert error == 0 and sizes == want["layer_sizes"]
+
+
 def test_budget_admission_refuses_when_full():
     g = GOLDEN[1]["graph"]  # n = 9 or 10
     with BatchEngine(max_slots=4, vram_budget=1) as b:
