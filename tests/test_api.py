@@ -4,6 +4,7 @@ reference's tests use duck types) and the command-line host's validation
 paths that run before any device is touched."""
 import hashlib
 import json
+import os
 import subprocess
 import sys
 import types
@@ -343,6 +344,18 @@ def test_graph_descriptor_is_built_once_and_overrides_copy_it():
         assert d.inverse_closed == int(gd.inverse_closed())  # the cached one is untouched
         assert list(flipped.permutations[:n * k]) == list(d.permutations[:n * k])
     assert prepare(prepared)[0] is prepared[0]
+
+
+@pytest.mark.parametrize("argv, env", [(["--gpus", "2"], {}), ([], {"WORLD_SIZE": "8"})])
+def test_bench_refuses_more_than_one_gpu(argv, env):
+    # One exact BFS runs on one GPU: a multi-GPU request exits with a message
+    # before touching a device.
+    bench = Path(__file__).resolve().parent.parent / "bench.py"
+    out = subprocess.run([sys.executable, str(bench), *argv], capture_output=True, text=True, timeout=120,
+                         env={**os.environ, **env})
+    assert out.returncode == 2
+    line = json.loads(out.stdout.strip().splitlines()[-1])
+    assert "single-GPU build" in line["error"]
 
 
 def test_serving_job_stream_mix_and_rate():
